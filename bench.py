@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Wilson Dslash (one GPU).
+
+Headline: FP32 4D Wilson Dslash on a 32^3 x 64 lattice, GFLOP/s at 1320
+flop/site, one "step" = one application over one checkerboard (Vh = 1,048,576
+target sites), steps alternating isign = +-1 and the source checkerboard, with
+gauge and spinors resident in HBM.  Inputs are synthetic: SU(3) links from
+Gram-Schmidt-reunitarised complex Gaussians (seed 1) and N(0,1) spinors,
+generated on the device.
+
+Also reported (same JSON line): the roofline of the dominant kernel measured
+with HIP events on its launch stream, the oracle's CPU throughput on the host
+cores (a bounded sample of the same lattice), and secondary figures (FP64, 3D,
+12-real links).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E nominal
+LATT = (32, 32, 32, 64)
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=True):
+    import torch
+    from sse_wilson_dslash_amd import F32, F64
+    from sse_wilson_dslash_amd.device import WilsonDslash
+
+    prec = {"f32": F32, "f64": F64}[prec_name]
+    op = WilsonDslash(LATT, nd=nd, prec=prec, gauge_reals=G)
+    op.gen_gauge(seed=1)
+    psi = op.gen_spinor(seed=1, field=0)
+    res = op.new_spinor(zero=True)
+    torch.cuda.synchronize()
+    op.time_apply(res, psi, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = op.time_apply(res, psi, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    flops = op.flops_per_call()
+    byts = op.bytes_per_call()
+    out = {
+        "ms": ms,
+        "gflops": flops / (ms * 1e-3) / 1e9,
+        "gbs": byts / (ms * 1e-3) / 1e9,
+        "bytes_per_call": byts,
+        "wall_s": wall,
+    }
+    if verbose:
+        log(f"[gpu] {prec_name} nd={nd} G={G}: {ms * 1e3:.1f} us/call, {out['gflops']:.0f} GFLOP/s, "
+            f"{out['gbs']:.0f} GB/s ({out['gbs'] / PEAK_HBM_GBS:.1%} of HBM peak)")
+    del op, psi, res
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_leg(seconds: float, threads: int):
+    """The oracle (plain C + OpenMP) on the host cores, same lattice, FP32."""
+    import numpy as np
+    from oracle import oracle
+    from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
+
+    used = oracle.set_threads(threads)
+    u = gen_gauge_host(LATT, 4, seed=1, dtype=np.float32)
+    psi = gen_spinor_host(LATT, 4, seed=1, dtype=np.float32)
+    res = np.zeros_like(psi)
+    oracle.dslash(u, psi, res, LATT, 1, 0)  # warm-up (first touch)
+    times = []
+    t_start = time.perf_counter()
+    k = 0
+    while True:
+        t0 = time.perf_counter()
+        oracle.dslash(u, psi, res, LATT, 1 if k % 4 < 2 else -1, k % 2)
+        times.append(time.perf_counter() - t0)
+        k += 1
+        if (time.perf_counter() - t_start > seconds and k >= 3) or k >= 200:
+            break
+    times.sort()
+    med = times[len(times) // 2]
+    vh = int(np.prod(LATT)) // 2
+    g = 1320 * vh / med / 1e9
+    log(f"[cpu] oracle f32 on {used} threads: {med * 1e3:.1f} ms/call, {g:.2f} GFLOP/s ({k} calls)")
+    return {"value": g, "unit": "GFLOP/s", "cores": used, "kind": "port",
+            "sample": f"{k} calls of the FP32 4D oracle on 32x32x32x64 (one checkerboard each), median "
+                      f"{med * 1e3:.1f} ms/call, OpenMP over target sites"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: the Dslash here runs on one GPU only "
+              "(multi-GPU halo exchange is not built); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+
+    log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    head = gpu_leg(args.steps, args.warmup, "f32", 4, 18)
+    extra = {}
+    if not args.no_extra:
+        n = max(200, args.steps // 4)
+        for key, (p, nd, G) in {"f64_4d": ("f64", 4, 18), "f32_4d_g12": ("f32", 4, 12),
+                                "f32_3d": ("f32", 3, 18), "f64_3d": ("f64", 3, 18)}.items():
+            r = gpu_leg(n, 20, p, nd, G)
+            extra[key] = {"gflops": round(r["gflops"], 1), "us_per_call": round(r["ms"] * 1e3, 2),
+                          "hbm_gbs": round(r["gbs"], 1), "hbm_frac": round(r["gbs"] / PEAK_HBM_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_leg(args.cpu_seconds, threads)
+
+    gbs = head["gbs"]
+    traffic = measured_traffic("dslash_wilson_f32_32c64_pmc.txt")
+    line = {
+        "metric": "Dslash GFLOP/s + % HBM roofline, 32^3x64; mixed vs FP64 CG time-to-1e-10",
+        "value": round(head["gflops"], 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SU(3) gauge, Gaussian spinors, generated on device)",
+        "config": {"workload": "wilson_dslash_4d_fp32_32x32x32x64", "lattice": list(LATT),
+                   "sites_per_step": int(32 * 32 * 32 * 64 // 2), "flop_per_site": 1320,
+                   "gauge_reals": 18, "steps_alternate": "isign +-1, source cb 0/1"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,152 @@
+/*
+ * dslash_mi355x.h -- device-resident C ABI of the MI355X Wilson Dslash.
+ *
+ * The legacy drop-in entry points (init_sse_su3dslash, sse_su3dslash_wilson,
+ * ...) are declared in sse_dslash.h / sse_dslash_3d.h and live in
+ * libsse_dslash_f32.so / libsse_dslash_f64.so.  This header is the layer those
+ * libraries are built on, exported by libdslash_mi355x.so.  It works on fields
+ * that already live in HBM, in the library's own layout:
+ *
+ *   spinor, one checkerboard:  [nvec][Vh] 16-byte vectors, the 24 reals of a
+ *     site in (spin, colour, re/im) order; nvec = 6 (F32), 12 (F64), 3 (F16,
+ *     BF16).  16-bit formats store x / max|x| per site and append one float
+ *     norm per site after the vectors.  Size: dslash_spinor_cb_bytes().
+ *     A full-lattice field is checkerboard 0 followed by checkerboard 1, each
+ *     dslash_spinor_cb_bytes() long.
+ *   gauge, one TARGET checkerboard:  for every target site x and direction mu
+ *     the pair (U_mu(x), U_mu(x-mu)^dagger) as 2*G reals (G = 18 full matrix,
+ *     G = 12 first two rows, third rebuilt as conj(row0 x row1)), stored as
+ *     [nd][pairvec][Vh] 16-byte vectors.  Size: dslash_gauge_cb_bytes().
+ *     A full gauge object is the bundle of target cb 0 followed by target cb 1.
+ *
+ * Site order within a checkerboard is the reference's canonical order:
+ * 4D cb2 (x fastest, lib/shift_tables_scalar.c:99-123) and 3D cb3d (t fastest,
+ * lib/shift_tables_scalar_3d.c:97-137).  The pack/unpack entry points convert
+ * from/to the legacy host layout (full lattice, [V][4][3][3][2] transposed links
+ * and [V][4][3][2] spinors, ordered by the caller's getLinearSiteIndex).
+ *
+ * All functions return 0 on success or a DSLASH_ERR_* code; the message of the
+ * last failure on the calling thread is dslash_last_error_message().  `stream`
+ * is a hipStream_t passed as void* (NULL = default stream).  Launches are
+ * asynchronous with respect to the host unless stated otherwise.
+ */
+#ifndef DSLASH_MI355X_H
+#define DSLASH_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Storage formats */
+#define DSLASH_F32 0
+#define DSLASH_F64 1
+#define DSLASH_F16 2  /* binary16 spinors/links, per-site float norm, FP32 math */
+#define DSLASH_BF16 3 /* bfloat16 spinors/links, per-site float norm, FP32 math */
+
+/* Status codes */
+#define DSLASH_OK 0
+#define DSLASH_ERR_INVALID 1     /* bad argument (shape, precision, isign, cb) */
+#define DSLASH_ERR_HIP 2         /* a HIP runtime call or launch failed */
+#define DSLASH_ERR_UNSUPPORTED 3 /* valid but not built (e.g. combination) */
+#define DSLASH_ERR_NOT_INIT 4    /* legacy API used before init */
+#define DSLASH_ERR_NOT_CONVERGED 5
+
+/* ---- status ---------------------------------------------------------- */
+int dslash_last_error(void);
+const char* dslash_last_error_message(void);
+int dslash_version(void);
+
+/* ---- sizes (host only, no device access) ------------------------------ */
+/* nd = 4 for the 4D operator, 3 for the spatial (3D) operator. */
+size_t dslash_spinor_cb_bytes(const int latt[4], int nd, int prec);
+size_t dslash_gauge_cb_bytes(const int latt[4], int nd, int prec, int gauge_reals);
+/* 0 if latt is valid for nd (positive; dims 0..nd-1 even), else DSLASH_ERR_INVALID */
+int dslash_check_lattice(const int latt[4], int nd);
+
+/* ---- layout conversion (device pointers) ------------------------------ */
+/* Legacy packed gauge [V][4][3][3][2] (prec_in F32/F64, caller order given by
+ * site_table: my full index -> caller linear index; NULL = canonical order)
+ * -> link bundle of one target checkerboard in prec_out / gauge_reals. */
+int dslash_pack_gauge(void* d_bundle, int target_cb, const void* d_u_packed, int prec_in,
+                      const int latt[4], int nd, int prec_out, int gauge_reals,
+                      const int* d_site_table, void* stream);
+/* Legacy full-lattice spinor [V][4][3][2] -> one checkerboard in device layout. */
+int dslash_pack_spinor(void* d_cb, int cb, const void* d_aos, int prec_in,
+                       const int latt[4], int nd, int prec_out,
+                       const int* d_site_table, void* stream);
+/* One checkerboard in device layout -> legacy AoS.  compact = 0 writes the
+ * sites at their caller positions in a full-lattice array; compact = 1 writes
+ * them densely in checkerboard order ([Vh][4][3][2]). */
+int dslash_unpack_spinor(void* d_aos, int cb, const void* d_cb, int prec_cb,
+                         const int latt[4], int nd, int prec_out,
+                         const int* d_site_table, int compact, void* stream);
+/* Precision conversion of one checkerboard (either direction, any formats). */
+int dslash_convert_spinor(void* d_out, int prec_out, const void* d_in, int prec_in,
+                          const int latt[4], int nd, void* stream);
+
+/* ---- the hot path ----------------------------------------------------- */
+/* res_cb(target) = D psi_cb(source = 1 - target_cb): the Wilson hopping term
+ *   sum_mu U_mu(x)(1 - isign gamma_mu) psi(x+mu) + U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
+ * over mu < nd (replaces sse_su3dslash_wilson's site loop,
+ * lib/sse_su3dslash_32bit_scalar.c:185-385).  psi_cb and res_cb are single
+ * checkerboards; gauge_cb is the bundle of target_cb.  Target sites are
+ * overwritten.  isign must be +1 or -1. */
+int dslash_apply(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb,
+                 const int latt[4], int nd, int prec, int gauge_reals,
+                 int isign, int target_cb, void* stream);
+
+/* Full-lattice form with the legacy meaning of cb (= SOURCE checkerboard):
+ * reads psi's cb half, overwrites res's 1-cb half, leaves res's cb half
+ * untouched (lib/sse_su3dslash_32bit_scalar.c:159-180).  d_gauge holds the
+ * bundles of both target checkerboards. */
+int dslash_apply_full(void* d_res, const void* d_psi, const void* d_gauge,
+                      const int latt[4], int nd, int prec, int gauge_reals,
+                      int isign, int cb, void* stream);
+
+/* Launch tuning: threads per block of the Dslash kernels (64, 128, 192 or 256)
+ * and whether blocks are remapped XCD-contiguously (1) or not (0). */
+int dslash_set_launch(int block_threads, int xcd_remap);
+int dslash_get_launch(int* block_threads, int* xcd_remap);
+
+/* ---- synthetic fields ------------------------------------------------- */
+/* Fill a legacy-layout gauge [V][4][3][3][2] (prec F32/F64, canonical order of
+ * nd) with SU(3) links: complex Gaussian 3x3 reunitarised by Gram-Schmidt,
+ * stored transposed as the reference's packer does
+ * (lib/qdp_packer_nopad.cc:7-19).  The link of a site depends only on
+ * (seed, lexicographic site, mu). */
+int dslash_gen_gauge_host(void* u_packed, int prec, const int latt[4], int nd, uint64_t seed);
+int dslash_gen_gauge_device(void* d_u_packed, int prec, const int latt[4], int nd, uint64_t seed, void* stream);
+/* Fill a legacy-layout full-lattice spinor with i.i.d. N(0,1) real/imag parts. */
+int dslash_gen_spinor_host(void* psi, int prec, const int latt[4], int nd, uint64_t seed, int field);
+int dslash_gen_spinor_device(void* d_psi, int prec, const int latt[4], int nd, uint64_t seed, int field, void* stream);
+
+/* The raw Philox4x32-10 block function behind the generators (exposed so its
+ * known-answer vectors can be checked on the host). */
+void dslash_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- canonical layout callbacks --------------------------------------- */
+/* What QDP++'s Layout supplies to init_sse_su3dslash in cb2 (nd = 4) or cb3d
+ * (nd = 3) mode, for callers without QDP++: set the lattice once, then pass
+ * these three functions as getSiteCoords / getLinearSiteIndex / nodeNumber.
+ * Process-global state, like QDP++'s Layout. */
+int dslash_layout_set(const int latt[4], int nd);
+int dslash_layout_linear_index(const int coord[]);
+void dslash_layout_site_coords(int coord[], int node, int linearsite);
+int dslash_layout_node_number(const int coord[]);
+
+/* ---- timing ----------------------------------------------------------- */
+/* Launch dslash_apply n times back to back on `stream` (alternating isign and
+ * target checkerboard when alternate != 0) bracketed by HIP events recorded on
+ * that same stream; returns the average milliseconds per launch in *ms. */
+int dslash_time_apply(void* d_res_full, const void* d_psi_full, const void* d_gauge_full,
+                      const int latt[4], int nd, int prec, int gauge_reals,
+                      int n, int alternate, void* stream, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DSLASH_MI355X_H */

@@ -1,0 +1,137 @@
+"""The drop-in API (sse_dslash.h / sse_dslash_3d.h) through libsse_dslash_f32.so /
+libsse_dslash_f64.so, as tests/testDslashFull.cc:61-112 and
+tests/testDslash3D.cc:64-113 exercise the reference: a reunitarised Gaussian
+gauge, a Gaussian source, both isign and both source checkerboards, compared
+against an independent Dslash (here the oracle)."""
+import numpy as np
+import pytest
+import torch
+
+from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
+from sse_wilson_dslash_amd.layout import volume
+from sse_wilson_dslash_amd.legacy import SSEDslash, table_callbacks
+from tests import refimpl as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+
+
+@pytest.fixture(scope="module")
+def libs():
+    return {32: SSEDslash(32), 64: SSEDslash(64)}
+
+
+def diff_norm(a, b, latt):
+    """tests/testDslashFull.cc:103-104: sqrt(norm2(diff)) / (4*3*2*V/2)."""
+    V = volume(latt)
+    return np.sqrt(np.sum((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)) / (24 * V / 2)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("nd", [4, 3])
+def test_legacy_canonical(oracle, libs, prec, nd):
+    lg = libs[prec]
+    dt = lg.dtype
+    latt = (8, 8, 8, 8) if nd == 4 else (8, 4, 6, 5)
+    print(f"[legacy canonical] prec={prec} nd={nd}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=1, dtype=dt)
+    psi = gen_spinor_host(latt, nd, seed=1, dtype=dt)
+    init = lg.init_sse_su3dslash if nd == 4 else lg.init_sse_su3dslash_3d
+    free = lg.free_sse_su3dslash if nd == 4 else lg.free_sse_su3dslash_3d
+    apply = lg.sse_su3dslash_wilson if nd == 4 else lg.sse_su3dslash_wilson_3d
+    init(latt)
+    try:
+        vh = volume(latt) // 2
+        for isign in (1, -1):
+            for cb in (0, 1):
+                res = np.full_like(psi, -2.0)
+                apply(u, psi, res, isign, cb)
+                want = oracle.dslash(u, psi, np.full_like(psi, -2.0), latt, isign, cb, nd=nd)
+                # the reference's own acceptance (1e-7 FP32, 1e-16 FP64 diff_norm) ...
+                assert diff_norm(res, want, latt) < (1e-7 if prec == 32 else 1e-16)
+                # ... and this project's relative bound
+                assert R.rel_err(res, want) < (1e-5 if prec == 32 else 1e-13)
+                assert np.all(res.reshape(-1, 24)[cb * vh:(cb + 1) * vh] == -2.0)
+        err = lg.last_error() if nd == 4 else lg.last_error_3d()
+        assert err == 0
+    finally:
+        free()
+
+
+@pytest.mark.parametrize("nd", [4, 3])
+def test_legacy_caller_order_callbacks(oracle, libs, nd):
+    """A non-canonical caller order supplied through the callbacks."""
+    lg = libs[64]
+    latt = (4, 4, 2, 6) if nd == 4 else (4, 2, 4, 3)
+    V = volume(latt)
+    perm = np.random.default_rng(5).permutation(V)  # my canonical index -> caller index
+    cbs = table_callbacks(latt, perm, nd)
+    u = gen_gauge_host(latt, nd, seed=3)
+    psi = gen_spinor_host(latt, nd, seed=3)
+    u_c = np.empty_like(u); u_c[perm] = u
+    p_c = np.empty_like(psi); p_c[perm] = psi
+    init = lg.init_sse_su3dslash if nd == 4 else lg.init_sse_su3dslash_3d
+    free = lg.free_sse_su3dslash if nd == 4 else lg.free_sse_su3dslash_3d
+    apply = lg.sse_su3dslash_wilson if nd == 4 else lg.sse_su3dslash_wilson_3d
+    init(latt, *cbs)
+    try:
+        for isign in (1, -1):
+            for cb in (0, 1):
+                res_c = np.full_like(psi, 9.0)
+                apply(u_c, p_c, res_c, isign, cb)
+                want = oracle.dslash(u, psi, np.full_like(psi, 9.0), latt, isign, cb, nd=nd)
+                assert R.rel_err(res_c[perm], want) < 1e-13
+    finally:
+        free()
+
+
+def test_legacy_semantics(oracle, libs):
+    """Aliasing psi == res, isign no-op, refcounting, resident-gauge paths."""
+    lg = libs[32]
+    latt = (4, 4, 4, 4)
+    u = gen_gauge_host(latt, 4, seed=1, dtype=np.float32)
+    psi = gen_spinor_host(latt, 4, seed=1, dtype=np.float32)
+    lg.init_sse_su3dslash(latt)
+    lg.init_sse_su3dslash((6, 6, 6, 6))  # refcount 2: size ignored (lib/sse_su3dslash_32bit_scalar.c:95-99)
+    try:
+        want = oracle.dslash(u, psi, psi.copy(), latt, 1, 0)
+        inplace = psi.copy()
+        lg.sse_su3dslash_wilson(u, inplace, inplace, 1, 0)
+        assert R.rel_err(inplace, want) < 1e-5
+        before = inplace.copy()
+        lg.sse_su3dslash_wilson(u, psi, inplace, 0, 0)  # isign not +-1: nothing happens
+        lg.sse_su3dslash_wilson(u, psi, inplace, 3, 1)
+        assert np.array_equal(inplace, before)
+        lg.sse_su3dslash_prepost_receives()
+        lg.free_sse_su3dslash()  # refcount 1: still usable
+        res = np.zeros_like(psi)
+        lg.sse_su3dslash_wilson(u, psi, res, -1, 1)
+        assert R.rel_err(res, oracle.dslash(u, psi, np.zeros_like(psi), latt, -1, 1)) < 1e-5
+        # resident gauge, 18 and 12 reals
+        for G in (18, 12):
+            assert lg.gauge_upload(u, G) == 0
+            for isign in (1, -1):
+                for cb in (0, 1):
+                    r2 = np.zeros_like(psi)
+                    assert lg.wilson_resident(psi, r2, isign, cb) == 0
+                    w = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb)
+                    assert R.rel_err(r2, w) < 1e-5
+        # device-resident spinors in the library layout
+        from sse_wilson_dslash_amd import F32
+        from sse_wilson_dslash_amd.device import WilsonDslash
+        op = WilsonDslash(latt, 4, F32, 12)
+        d_psi = op.pack_spinor(psi)
+        d_res = op.new_spinor(zero=True)
+        assert lg.wilson_device(d_res, d_psi, 1, 1, torch.cuda.current_stream().cuda_stream) == 0
+        torch.cuda.synchronize()
+        got = op.unpack_spinor(d_res, dtype=np.float32)
+        assert R.rel_err(got, oracle.dslash(u, psi, np.zeros_like(psi), latt, 1, 1)) < 1e-5
+    finally:
+        lg.free_sse_su3dslash()
+    # after the last free the API reports use-before-init instead of crashing
+    assert lg.wilson_resident(psi, np.zeros_like(psi), 1, 0) != 0

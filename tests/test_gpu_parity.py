@@ -1,0 +1,157 @@
+"""GPU parity of the fused HIP Dslash with the oracle, through the C ABI.
+
+Tolerances (relative L2 over the target checkerboard, and max-abs relative to
+the largest entry): FP64 1e-13, FP32 1e-5.  The 16-bit formats are a storage
+compression of the operator used inside the mixed-precision solver; their
+error against the FP64 oracle is bounded and reported, not a parity claim.
+"""
+import numpy as np
+import pytest
+import torch
+
+from sse_wilson_dslash_amd import F16, F32, F64, BF16
+from sse_wilson_dslash_amd.device import WilsonDslash, gen_gauge_host, gen_spinor_host
+from sse_wilson_dslash_amd.layout import volume
+from tests import refimpl as R
+
+pytestmark = pytest.mark.gpu
+
+TOL = {F64: 1e-13, F32: 1e-5}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+
+
+def run_gpu(u, psi, latt, nd, prec, G, isign, cb, sentinel=None):
+    op = WilsonDslash(latt, nd=nd, prec=prec, gauge_reals=G)
+    op.load_gauge(u)
+    d_psi = op.pack_spinor(psi)
+    d_res = op.pack_spinor(sentinel if sentinel is not None else np.zeros_like(psi))
+    op.apply(d_res, d_psi, isign, cb)
+    torch.cuda.synchronize()
+    dt = np.float64 if prec == F64 else np.float32
+    return op.unpack_spinor(d_res, dtype=dt)
+
+
+def check(got, want, latt, cb, prec, label=""):
+    vh = volume(latt) // 2
+    t = slice((1 - cb) * vh, (2 - cb) * vh)
+    g = got.reshape(-1, 24)[t]
+    w = want.reshape(-1, 24)[t]
+    e2, einf = R.rel_err(g, w), R.max_rel_err(g, w)
+    tol = TOL[prec]
+    assert e2 < tol and einf < tol, f"{label}: rel L2 {e2:.3e}, rel max {einf:.3e} (tol {tol})"
+    return e2
+
+
+@pytest.mark.parametrize("nd,latt", [(4, (4, 4, 4, 4)), (4, (2, 6, 4, 2)), (4, (6, 2, 2, 10)),
+                                     (3, (4, 4, 4, 4)), (3, (4, 2, 6, 3)), (3, (2, 2, 2, 5))])
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("G", [18, 12])
+def test_small_lattices(oracle, nd, latt, prec, G):
+    dt = np.float64 if prec == F64 else np.float32
+    print(f"[parity small] nd={nd} latt={latt} prec={prec} G={G}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=1, dtype=dt)
+    psi = gen_spinor_host(latt, nd, seed=1, field=0, dtype=dt)
+    for isign in (1, -1):
+        for cb in (0, 1):
+            sentinel = np.full_like(psi, 5.5)
+            got = run_gpu(u, psi, latt, nd, prec, G, isign, cb, sentinel)
+            want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
+            check(got, want, latt, cb, prec, f"nd={nd} {latt} isign={isign} cb={cb}")
+            vh = volume(latt) // 2
+            assert np.all(got.reshape(-1, 24)[cb * vh:(cb + 1) * vh] == 5.5), "source cb of res modified"
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_non_unitary_gauge_full_links(oracle, prec):
+    """With 18-real links the operator is exact for any complex matrices (the
+    reference never assumes unitarity); the force checks rely on this."""
+    latt = (4, 2, 4, 6)
+    rng = np.random.default_rng(4)
+    V = volume(latt)
+    dt = np.float64 if prec == F64 else np.float32
+    u = rng.normal(size=(V, 4, 3, 3, 2)).astype(dt)
+    psi = rng.normal(size=(V, 4, 3, 2)).astype(dt)
+    for nd in (4, 3):
+        for isign in (1, -1):
+            for cb in (0, 1):
+                got = run_gpu(u, psi, latt, nd, prec, 18, isign, cb)
+                want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
+                check(got, want, latt, cb, prec)
+
+
+@pytest.mark.parametrize("nd", [4, 3])
+@pytest.mark.parametrize("prec,G", [(F32, 18), (F64, 18), (F32, 12)])
+def test_full_size_32c64(oracle, nd, prec, G):
+    """Config: 32^3 x 64, both parities and both isign, against the oracle."""
+    latt = (32, 32, 32, 64)
+    dt = np.float64 if prec == F64 else np.float32
+    print(f"[parity 32^3x64] nd={nd} prec={prec} G={G}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=1, dtype=dt)
+    psi = gen_spinor_host(latt, nd, seed=1, field=0, dtype=dt)
+    op = WilsonDslash(latt, nd=nd, prec=prec, gauge_reals=G)
+    op.load_gauge(u)
+    d_psi = op.pack_spinor(psi)
+    d_res = op.new_spinor(zero=True)
+    want = np.empty_like(psi)
+    for isign in (1, -1):
+        for cb in (0, 1):
+            op.apply(d_res, d_psi, isign, cb)
+            got = op.unpack_spinor(d_res, dtype=dt, cbs=(1 - cb,))
+            oracle.dslash(u, psi, want, latt, isign, cb, nd=nd)
+            e = check(got, want, latt, cb, prec, f"isign={isign} cb={cb}")
+            print(f"  isign={isign:+d} cb={cb} rel_err={e:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("prec,tol", [(F16, 2e-3), (BF16, 1.6e-2)])
+def test_half_storage_operator_error(oracle, prec, tol):
+    """16-bit spinors (per-site norm) and 12-real 16-bit links, FP32 math:
+    error against the FP64 oracle is bounded by the storage rounding."""
+    latt = (8, 8, 8, 8)
+    u = gen_gauge_host(latt, 4, seed=1)
+    psi = gen_spinor_host(latt, 4, seed=1)
+    for isign in (1, -1):
+        for cb in (0, 1):
+            got = run_gpu(u, psi, latt, 4, prec, 12, isign, cb)
+            want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb)
+            vh = volume(latt) // 2
+            t = slice((1 - cb) * vh, (2 - cb) * vh)
+            e = R.rel_err(got.reshape(-1, 24)[t], want.reshape(-1, 24)[t])
+            print(f"  prec={prec} isign={isign:+d} cb={cb} rel_err vs FP64 oracle = {e:.3e}", flush=True)
+            assert e < tol
+
+
+def test_device_generators_match_host():
+    latt = (8, 4, 6, 4)
+    op = WilsonDslash(latt, nd=4, prec=F64)
+    for nd in (4, 3):
+        lib = op.lib
+        from sse_wilson_dslash_amd._lib import latt_arg, ptr, check as ck
+        V = volume(latt)
+        d = torch.empty(V * 72, dtype=torch.float64, device="cuda")
+        ck(lib.dslash_gen_gauge_device(ptr(d), F64, latt_arg(latt), nd, 1, None))
+        h = gen_gauge_host(latt, nd, seed=1)
+        assert np.max(np.abs(d.cpu().numpy() - h.ravel())) < 1e-13
+        s = torch.empty(V * 24, dtype=torch.float64, device="cuda")
+        ck(lib.dslash_gen_spinor_device(ptr(s), F64, latt_arg(latt), nd, 1, 3, None))
+        hs = gen_spinor_host(latt, nd, seed=1, field=3)
+        assert np.max(np.abs(s.cpu().numpy() - hs.ravel())) < 1e-12
+
+
+@pytest.mark.parametrize("block,remap", [(64, 0), (128, 1), (192, 1), (256, 0)])
+def test_launch_shapes_agree(oracle, block, remap):
+    from sse_wilson_dslash_amd import _lib
+    latt = (8, 6, 4, 10)
+    u = gen_gauge_host(latt, 4, seed=2, dtype=np.float32)
+    psi = gen_spinor_host(latt, 4, seed=2, dtype=np.float32)
+    _lib.check(_lib.core().dslash_set_launch(block, remap))
+    try:
+        got = run_gpu(u, psi, latt, 4, F32, 18, 1, 0)
+    finally:
+        _lib.check(_lib.core().dslash_set_launch(256, 1))
+    want = oracle.dslash(u, psi, np.zeros_like(psi), latt, 1, 0)
+    check(got, want, latt, 0, F32)

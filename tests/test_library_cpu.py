@@ -1,0 +1,178 @@
+"""CPU-side checks of the HIP libraries: they load, export every declared
+symbol, and their host-only entry points (generators, layout callbacks, the C
+gauge packer, argument validation) behave as specified.  No kernel runs here."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from sse_wilson_dslash_amd import _lib
+from sse_wilson_dslash_amd.layout import coords_of_full, full_of_coords, lex_permutation, volume
+from tests import refimpl as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header: str) -> list[str]:
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", text)
+    return sorted(set(n for n in names if n not in ("if", "sizeof")))
+
+
+def test_core_exports_every_declared_symbol():
+    lib = _lib.core()
+    names = declared("dslash_mi355x.h")
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert all(n in _lib.PROTOTYPES for n in names), [n for n in names if n not in _lib.PROTOTYPES]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_legacy_exports_every_declared_symbol(prec):
+    _lib.core()
+    lib = C.CDLL(_lib.lib_path(_lib.LEGACY[prec]), mode=C.RTLD_LOCAL)
+    names = declared("sse_dslash.h") + declared("sse_dslash_3d.h")
+    for ref in ("init_sse_su3dslash", "free_sse_su3dslash", "sse_su3dslash_wilson",
+                "sse_su3dslash_prepost_receives", "init_sse_su3dslash_3d", "free_sse_su3dslash_3d",
+                "sse_su3dslash_wilson_3d", "pack_gauge_field"):
+        assert ref in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_philox_known_answers():
+    """Random123 known-answer vectors for philox4x32 with 10 rounds."""
+    lib = _lib.core()
+
+    def ph(ctr, key):
+        o = (C.c_uint32 * 4)()
+        lib.dslash_philox4x32_10((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), o)
+        return list(o)
+
+    assert ph([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert ph([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert ph([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+@pytest.mark.parametrize("nd", [4, 3])
+def test_generated_gauge_is_su3_and_layout_independent(nd):
+    from sse_wilson_dslash_amd.device import gen_gauge_host
+    latt = (4, 4, 2, 6)
+    u = gen_gauge_host(latt, nd, seed=1)
+    U = R.links(u)
+    V = volume(latt)
+    eye = np.eye(3)
+    UU = U @ U.conj().transpose(0, 1, 3, 2)
+    assert np.max(np.abs(UU - eye)) < 1e-14
+    assert np.max(np.abs(np.linalg.det(U) - 1)) < 1e-14
+    # the generator follows tests/reunit.cc:184-241: column 2 = conj(col0 x col1)
+    c0, c1, c2 = U[..., :, 0], U[..., :, 1], U[..., :, 2]
+    assert np.max(np.abs(c2 - np.cross(c0, c1).conj())) < 1e-14
+    # keyed by the lexicographic site: the 4D and 3D orders hold the same field
+    other = gen_gauge_host(latt, 7 - nd, seed=1)
+    lex_a = lex_permutation(latt, nd)
+    lex_b = lex_permutation(latt, 7 - nd)
+    a = np.empty_like(u); a[lex_a] = u
+    b = np.empty_like(u); b[lex_b] = other
+    assert np.array_equal(a, b)
+    # a different seed gives a different field
+    assert not np.array_equal(gen_gauge_host(latt, nd, seed=2), u)
+    # f32 generation is the f64 field rounded
+    assert np.array_equal(gen_gauge_host(latt, nd, seed=1, dtype=np.float32), u.astype(np.float32))
+    assert u.shape == (V, 4, 3, 3, 2)
+
+
+def test_generated_spinor_is_standard_gaussian():
+    from sse_wilson_dslash_amd.device import gen_spinor_host
+    latt = (8, 8, 8, 8)
+    p = gen_spinor_host(latt, 4, seed=1, field=0)
+    x = p.ravel()
+    assert abs(x.mean()) < 0.01 and abs(x.var() - 1) < 0.01
+    q = gen_spinor_host(latt, 4, seed=1, field=1)
+    assert abs(np.corrcoef(x, q.ravel())[0, 1]) < 0.01
+
+
+@pytest.mark.parametrize("nd,latt", [(4, (6, 10, 2, 12)), (4, (48, 2, 2, 6)), (3, (4, 6, 2, 96)),
+                                     (3, (24, 2, 4, 9)), (4, (2, 2, 2, 2))])
+def test_layout_callbacks_match_canonical_order(nd, latt):
+    """The exported cb2/cb3d callbacks (which use the kernels' fast division)
+    agree with the defining formulas for every site."""
+    lib = _lib.core()
+    _lib.check(lib.dslash_layout_set(_lib.latt_arg(latt), nd))
+    V = volume(latt)
+    c = coords_of_full(latt, nd, np.arange(V))
+    coord = (C.c_int * 4)()
+    for i in range(V):
+        lib.dslash_layout_site_coords(coord, 0, i)
+        assert list(coord) == list(c[i]), i
+        assert lib.dslash_layout_linear_index(coord) == i
+    assert np.array_equal(full_of_coords(latt, nd, c), np.arange(V))
+
+
+def test_sizes_and_validation():
+    lib = _lib.core()
+    L = _lib.latt_arg((32, 32, 32, 64))
+    vh = 32 * 32 * 32 * 64 // 2
+    assert lib.dslash_spinor_cb_bytes(L, 4, _lib.F32) == vh * 96
+    assert lib.dslash_spinor_cb_bytes(L, 4, _lib.F64) == vh * 192
+    assert lib.dslash_spinor_cb_bytes(L, 4, _lib.F16) == vh * 52
+    assert lib.dslash_gauge_cb_bytes(L, 4, _lib.F32, 18) == vh * 8 * 18 * 4
+    assert lib.dslash_gauge_cb_bytes(L, 4, _lib.F32, 12) == vh * 8 * 12 * 4
+    assert lib.dslash_gauge_cb_bytes(L, 3, _lib.F64, 18) == vh * 6 * 18 * 8
+    assert lib.dslash_gauge_cb_bytes(L, 4, _lib.F16, 12) == vh * 8 * 12 * 2
+    assert lib.dslash_check_lattice(_lib.latt_arg((4, 4, 4, 3)), 4) == _lib.ERR_INVALID
+    assert b"even" in lib.dslash_last_error_message()
+    assert lib.dslash_check_lattice(_lib.latt_arg((4, 4, 4, 3)), 3) == 0
+    assert lib.dslash_check_lattice(_lib.latt_arg((4, 4, 4, 4)), 5) == _lib.ERR_INVALID
+    # invalid isign / cb are rejected before any device access
+    st = lib.dslash_apply(1, 2, 3, _lib.latt_arg((4, 4, 4, 4)), 4, _lib.F32, 18, 0, 0, None)
+    assert st == _lib.ERR_INVALID and b"isign" in lib.dslash_last_error_message()
+    st = lib.dslash_apply(1, 2, 3, _lib.latt_arg((4, 4, 4, 4)), 4, _lib.F32, 18, 1, 2, None)
+    assert st == _lib.ERR_INVALID
+    st = lib.dslash_apply(1, 2, 3, _lib.latt_arg((4, 4, 4, 4)), 4, 9, 18, 1, 0, None)
+    assert st == _lib.ERR_INVALID
+    st = lib.dslash_apply(1, 2, 3, _lib.latt_arg((4, 4, 4, 4)), 4, _lib.F32, 10, 1, 0, None)
+    assert st == _lib.ERR_INVALID
+    assert lib.dslash_set_launch(100, 1) == _lib.ERR_INVALID
+    assert lib.dslash_set_launch(128, 0) == 0
+    b, r = C.c_int(), C.c_int()
+    lib.dslash_get_launch(C.byref(b), C.byref(r))
+    assert (b.value, r.value) == (128, 0)
+    assert lib.dslash_set_launch(256, 1) == 0
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_c_gauge_packer(prec):
+    """pack_gauge_field reorders [mu][cb][Vh] -> [cb][Vh][mu] without transposing
+    (lib/packer_nopad.c:29-45); unpack_gauge_field is its inverse."""
+    from sse_wilson_dslash_amd.legacy import SSEDslash
+    lg = SSEDslash(prec)
+    vh = 12
+    dt = np.float32 if prec == 32 else np.float64
+    u = np.arange(4 * 2 * vh * 18, dtype=dt).reshape(4, 2, vh, 18)
+    out = np.zeros_like(u).reshape(2, vh, 4, 18)
+    lg.pack_gauge_field(vh, u, out)
+    assert np.array_equal(out, u.transpose(1, 2, 0, 3))
+    back = np.zeros_like(u)
+    lg.unpack_gauge_field(vh, out, back)
+    assert np.array_equal(back, u)
+
+
+@pytest.mark.parametrize("nd,latt,msg", [(4, "[4,4,3,4]", b"4 dimensions"), (3, "[4,3,4,5]", b"3 dimensions")])
+def test_legacy_odd_dimension_exits_1(nd, latt, msg):
+    """lib/sse_su3dslash_32bit_scalar.c:103-110: odd global dimension -> stderr + exit(1)."""
+    fn = "init_sse_su3dslash" if nd == 4 else "init_sse_su3dslash_3d"
+    code = (f"import sys; sys.path.insert(0, {ROOT!r})\n"
+            "from sse_wilson_dslash_amd.legacy import SSEDslash\n"
+            f"SSEDslash(64).{fn}({latt})\n"
+            "print('not reached')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, timeout=120)
+    assert r.returncode == 1
+    assert msg in r.stderr and b"not reached" not in r.stdout

@@ -98,6 +98,13 @@ int dslash_apply(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb,
                  const int latt[4], int nd, int prec, int gauge_reals,
                  int isign, int target_cb, void* stream);
 
+/* 4D only: res_cb = x_cb + a * D psi_cb in one pass (x_cb may alias res_cb).
+ * With a = -kappa this is the Wilson matrix M = 1 - kappa D between
+ * checkerboards; the solvers use it for 1 - kappa^2 D_eo D_oe. */
+int dslash_apply_xpay(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb,
+                      const void* d_x_cb, double a, const int latt[4], int prec,
+                      int gauge_reals, int isign, int target_cb, void* stream);
+
 /* Full-lattice form with the legacy meaning of cb (= SOURCE checkerboard):
  * reads psi's cb half, overwrites res's 1-cb half, leaves res's cb half
  * untouched (lib/sse_su3dslash_32bit_scalar.c:159-180).  d_gauge holds the
@@ -106,10 +113,103 @@ int dslash_apply_full(void* d_res, const void* d_psi, const void* d_gauge,
                       const int latt[4], int nd, int prec, int gauge_reals,
                       int isign, int cb, void* stream);
 
-/* Launch tuning: threads per block of the Dslash kernels (64, 128, 192 or 256)
- * and whether blocks are remapped XCD-contiguously (1) or not (0). */
-int dslash_set_launch(int block_threads, int xcd_remap);
-int dslash_get_launch(int* block_threads, int* xcd_remap);
+/* Launch shape of the single-RHS kernels for (nd, prec, gauge_reals): threads
+ * per block (64, 128, 192 or 256) and whether blocks are remapped so each XCD
+ * walks a contiguous range of sites (1) or not (0).  nd = 0 sets every entry.
+ * dslash_tune_apply (below) measures and sets the best entry. */
+int dslash_set_launch(int nd, int prec, int gauge_reals, int block_threads, int xcd_remap);
+int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int* xcd_remap);
+
+/* ---- many right-hand sides, many lattices ----------------------------- */
+/* One lattice of a batched launch: nrhs single-checkerboard fields on one
+ * gauge, field k at d_psi + k*field_stride bytes (likewise d_res). */
+typedef struct {
+  int latt[4];
+  const void* d_gauge_cb; /* link bundle of target_cb */
+  const void* d_psi;      /* nrhs source fields (checkerboard 1 - target_cb) */
+  void* d_res;            /* nrhs target fields, must not overlap d_psi */
+  size_t field_stride;    /* bytes, multiple of 16, >= dslash_spinor_cb_bytes */
+  int nrhs;
+  int isign;
+  int target_cb;
+} dslash_mrhs_problem;
+
+/* 4D Dslash of every field of every problem (1..8 problems, F32 or F64) in ONE
+ * launch.  A thread owns one site and rhs_block (1, 2 or 4) fields and reuses
+ * each link for all of them.  grid_blocks = 0 launches one block per tile;
+ * grid_blocks > 0 launches a persistent grid of that many blocks that strides
+ * over the tiles of all lattices.  rhs_block / block_threads = 0 take the
+ * tuned (or default) values. */
+int dslash_apply_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, int gauge_reals,
+                      int rhs_block, int block_threads, int grid_blocks, void* stream);
+
+/* ---- autotuning -------------------------------------------------------- */
+/* Time every launch shape of the single-RHS kernel (block size x XCD remap) on
+ * the given fields (dslash_apply_full semantics, source cb 0; res is
+ * overwritten), keep the fastest for (nd, prec, gauge_reals) and report it. */
+int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec,
+                      int gauge_reals, void* stream, int* block_threads, int* xcd_remap, float* best_ms);
+/* Time rhs_block x block_threads (x persistent grid sizes when persistent != 0)
+ * for a batched launch; keeps the fastest for (prec, gauge_reals, persistent). */
+int dslash_tune_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, int gauge_reals, int persistent,
+                     void* stream, int* rhs_block, int* block_threads, int* grid_blocks, float* best_ms);
+/* Tuned choices as text lines; load applies them (missing file = DSLASH_ERR_INVALID). */
+int dslash_tune_save(const char* path);
+int dslash_tune_load(const char* path);
+
+/* ---- even-odd preconditioned Wilson CG (mixed precision) -------------- */
+/* Solves A x = b on the even checkerboard, A = Mhat^dagger Mhat,
+ * Mhat = 1 - kappa^2 D_eo D_oe, M = 1 - kappa D.  x and b are even-cb F64
+ * fields in the device layout.  The inner iteration runs in prec_sloppy
+ * (F16/BF16 spinors with per-site norms and 16-bit links, FP32 math; or F32;
+ * or F64 = plain FP64 CG); the residual is recomputed in FP64 at reliable
+ * updates and convergence is declared on that FP64 true residual. */
+typedef struct {
+  double kappa;           /* hopping parameter of M = 1 - kappa D */
+  double tol;             /* stop when ||b - A x|| / ||b|| <= tol (FP64) */
+  double delta;           /* reliable update when |r| < delta * max|r| since the last one;
+                             <= 0: only to confirm convergence */
+  int max_iter;           /* inner iterations */
+  int prec_sloppy;        /* DSLASH_F16, DSLASH_BF16, DSLASH_F32 or DSLASH_F64 */
+  int gauge_reals_sloppy; /* 18 or 12 */
+  int use_graph;          /* replay the inner iteration as a captured hipGraph */
+  int zero_guess;         /* start from x = 0 instead of the contents of x */
+} dslash_cg_params;
+
+typedef struct {
+  int iterations;         /* inner iterations (one A application each) */
+  int reliable_updates;   /* FP64 residual recomputations */
+  int converged;
+  double true_residual;   /* ||b - A x|| / ||b|| of the returned x, in FP64 */
+  double seconds;         /* wall time of the solve, device synchronised */
+} dslash_cg_result;
+
+/* Allocates the workspace and link storage; NULL on failure (see last error). */
+void* dslash_cg_create(const int latt[4], const dslash_cg_params* params);
+/* Packs a legacy-layout gauge (device pointer, F32/F64, canonical cb2 order)
+ * into FP64 18-real links and the sloppy links. */
+int dslash_cg_load_gauge(void* solver, const void* d_u_packed, int prec_in, void* stream);
+/* Synchronous.  Returns DSLASH_ERR_NOT_CONVERGED if tol was not reached. */
+int dslash_cg_solve(void* solver, void* d_x, const void* d_b, dslash_cg_result* result);
+/* out = A in (even cb): FP64 operator (sloppy = 0) or the sloppy one (fields in
+ * the sloppy format).  Synchronous. */
+int dslash_cg_apply_op(void* solver, void* d_out, const void* d_in, int sloppy);
+/* Change kappa, tol, delta, max_iter, use_graph, zero_guess (not the formats). */
+int dslash_cg_set_params(void* solver, const dslash_cg_params* params);
+int dslash_cg_destroy(void* solver);
+
+/* ---- backward (gauge force) -------------------------------------------- */
+/* Link gradient of S = Re <chi, D psi>, D from source checkerboard cb to 1-cb
+ * with the given isign (4D, FP64).  chi, psi: full-lattice F64 device fields
+ * (chi's target half and psi's source half are read).  Outputs, each
+ * [V][4][3][3][2] doubles in canonical cb2 order (either may be NULL):
+ *   d_grad_packed  dS / d(real entries of the packed, transposed link array);
+ *   d_force        the su(3) projection F = TA((U G)^dagger) (row-major), with
+ *                  dS = Re tr(X^dagger F) for dU = X U, X in su(3); needs
+ *                  d_u_packed (F64 packed links, canonical order).
+ * The gradient with respect to psi is D^dagger chi (dslash_apply with -isign). */
+int dslash_force(void* d_grad_packed, void* d_force, const void* d_u_packed, const void* d_chi,
+                 const void* d_psi, const int latt[4], int isign, int cb, void* stream);
 
 /* ---- synthetic fields ------------------------------------------------- */
 /* Fill a legacy-layout gauge [V][4][3][3][2] (prec F32/F64, canonical order of

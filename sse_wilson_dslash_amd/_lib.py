@@ -22,8 +22,33 @@ _ip = C.POINTER(C.c_int)
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 
+class MrhsProblem(C.Structure):
+    """dslash_mrhs_problem"""
+    _fields_ = [("latt", C.c_int * 4), ("d_gauge_cb", C.c_void_p), ("d_psi", C.c_void_p), ("d_res", C.c_void_p),
+                ("field_stride", C.c_size_t), ("nrhs", C.c_int), ("isign", C.c_int), ("target_cb", C.c_int)]
+
+
+class CgParams(C.Structure):
+    """dslash_cg_params"""
+    _fields_ = [("kappa", C.c_double), ("tol", C.c_double), ("delta", C.c_double), ("max_iter", C.c_int),
+                ("prec_sloppy", C.c_int), ("gauge_reals_sloppy", C.c_int), ("use_graph", C.c_int),
+                ("zero_guess", C.c_int)]
+
+
+class CgResult(C.Structure):
+    """dslash_cg_result"""
+    _fields_ = [("iterations", C.c_int), ("reliable_updates", C.c_int), ("converged", C.c_int),
+                ("true_residual", C.c_double), ("seconds", C.c_double)]
+
+
 # name -> (restype, argtypes); mirrors include/dslash_mi355x.h
 PROTOTYPES = {
+    "dslash_cg_create": (_vp, [_ip, C.POINTER(CgParams)]),
+    "dslash_cg_load_gauge": (_i, [_vp, _vp, _i, _vp]),
+    "dslash_cg_solve": (_i, [_vp, _vp, _vp, C.POINTER(CgResult)]),
+    "dslash_cg_apply_op": (_i, [_vp, _vp, _vp, _i]),
+    "dslash_cg_set_params": (_i, [_vp, C.POINTER(CgParams)]),
+    "dslash_cg_destroy": (_i, [_vp]),
     "dslash_last_error": (_i, []),
     "dslash_last_error_message": (C.c_char_p, []),
     "dslash_version": (_i, []),
@@ -36,8 +61,15 @@ PROTOTYPES = {
     "dslash_convert_spinor": (_i, [_vp, _i, _vp, _i, _ip, _i, _vp]),
     "dslash_apply": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _i, _i, _vp]),
     "dslash_apply_full": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _i, _i, _vp]),
-    "dslash_set_launch": (_i, [_i, _i]),
-    "dslash_get_launch": (_i, [_ip, _ip]),
+    "dslash_apply_xpay": (_i, [_vp, _vp, _vp, _vp, C.c_double, _ip, _i, _i, _i, _i, _vp]),
+    "dslash_set_launch": (_i, [_i, _i, _i, _i, _i]),
+    "dslash_get_launch": (_i, [_i, _i, _i, _ip, _ip]),
+    "dslash_apply_mrhs": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp]),
+    "dslash_tune_apply": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _vp, _ip, _ip, _fp]),
+    "dslash_tune_mrhs": (_i, [_vp, _i, _i, _i, _i, _vp, _ip, _ip, _ip, _fp]),
+    "dslash_force": (_i, [_vp, _vp, _vp, _vp, _vp, _ip, _i, _i, _vp]),
+    "dslash_tune_save": (_i, [C.c_char_p]),
+    "dslash_tune_load": (_i, [C.c_char_p]),
     "dslash_gen_gauge_host": (_i, [_vp, _i, _ip, _i, _u64]),
     "dslash_gen_gauge_device": (_i, [_vp, _i, _ip, _i, _u64, _vp]),
     "dslash_gen_spinor_host": (_i, [_vp, _i, _ip, _i, _u64, _i]),

@@ -1,0 +1,22 @@
+// Internal (C++) entry points shared between the translation units of
+// libdslash_mi355x.so.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace wd {
+
+// Launch shape of the single-RHS Dslash kernels for one (nd, format, link reals).
+struct LaunchCfg {
+  int block;  // threads per block
+  int remap;  // XCD-contiguous block order
+};
+LaunchCfg launch_cfg(int nd, int prec, int G);
+void set_launch_cfg(int nd, int prec, int G, LaunchCfg c);
+
+// res_cb = D psi_cb (xp_cb == nullptr) or res_cb = xp_cb + a * D psi_cb, all on
+// one checkerboard in the device layout of `prec`; xp_cb may alias res_cb.
+int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd, int prec, int G,
+                int isign, int tcb, const void* xp_cb, double a, hipStream_t s);
+
+}  // namespace wd

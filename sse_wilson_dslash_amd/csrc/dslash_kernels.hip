@@ -27,6 +27,7 @@
 #include "dslash_mi355x.h"
 #include "philox.h"
 #include "status.h"
+#include "dslash_internal.h"
 
 namespace wd {
 
@@ -52,8 +53,14 @@ void clear_error() {
 // ---------------------------------------------------------------------------
 // launch configuration
 // ---------------------------------------------------------------------------
-static int g_block = 256;
-static int g_remap = 1;
+// Threads per block and XCD remapping per (4D/3D, format, link reals); set by
+// dslash_set_launch (all entries) or by the autotuner (one entry).
+static LaunchCfg g_cfg[2][4][2] = {
+    {{{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}},
+    {{{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}}};
+
+LaunchCfg launch_cfg(int nd, int prec, int G) { return g_cfg[nd == 4][prec & 3][G == 12]; }
+void set_launch_cfg(int nd, int prec, int G, LaunchCfg c) { g_cfg[nd == 4][prec & 3][G == 12] = c; }
 
 static int check_lattice(const int latt[4], int nd) {
   WD_REQUIRE(latt != nullptr, "latt is NULL");
@@ -82,9 +89,13 @@ struct ApplyArgs {
   v4u* res;
   float* res_norm;
   const v4u* gauge;
+  const v4u* xp;  // optional target-cb field of the epilogue res = xp + a * D psi
+  const float* xp_norm;
+  double a;
   Geom g;
   int tcb;
   int remap;
+  int block;
 };
 
 template <int ND, int P, int G, int IS, int MU>
@@ -107,7 +118,7 @@ __device__ __forceinline__ void hop_dir(const ApplyArgs& a, const Coord& c, uint
   }
 }
 
-template <int ND, int P, int G, int IS>
+template <int ND, int P, int G, int IS, bool XPAY>
 __global__ void __launch_bounds__(256) k_dslash(ApplyArgs a) {
   typedef typename Fmt<P>::compute T;
   uint32_t b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -123,13 +134,30 @@ __global__ void __launch_bounds__(256) k_dslash(ApplyArgs a) {
   hop_dir<ND, P, G, IS, 1>(a, c, idx, acc);
   hop_dir<ND, P, G, IS, 2>(a, c, idx, acc);
   if constexpr (ND == 4) hop_dir<ND, P, G, IS, 3>(a, c, idx, acc);
-  store_spinor<P, true>(a.res, a.res_norm, idx, a.g.vh, acc);
+  if constexpr (XPAY) {
+    Spinor<T> x = load_spinor<P>(a.xp, a.xp_norm, idx, a.g.vh);
+    const T s = (T)a.a;
+#pragma unroll
+    for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        acc.s[sp][k].re = x.s[sp][k].re + s * acc.s[sp][k].re;
+        acc.s[sp][k].im = x.s[sp][k].im + s * acc.s[sp][k].im;
+      }
+    store_spinor<P, false>(a.res, a.res_norm, idx, a.g.vh, acc);
+  } else {
+    store_spinor<P, true>(a.res, a.res_norm, idx, a.g.vh, acc);
+  }
 }
 
 template <int ND, int P, int G, int IS>
 static void launch_dslash_t(const ApplyArgs& a, hipStream_t s) {
-  uint32_t blocks = (a.g.vh + g_block - 1) / g_block;
-  hipLaunchKernelGGL((k_dslash<ND, P, G, IS>), dim3(blocks), dim3(g_block), 0, s, a);
+  uint32_t blocks = (a.g.vh + a.block - 1) / a.block;
+  if (a.xp) {
+    if constexpr (ND == 4) hipLaunchKernelGGL((k_dslash<ND, P, G, IS, true>), dim3(blocks), dim3(a.block), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_dslash<ND, P, G, IS, false>), dim3(blocks), dim3(a.block), 0, s, a);
+  }
 }
 
 template <int ND, int P, int G>
@@ -158,8 +186,8 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
   }
 }
 
-static int apply_cb(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd,
-                    int prec, int G, int isign, int tcb, hipStream_t s) {
+int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd, int prec, int G,
+                int isign, int tcb, const void* xp_cb, double xa, hipStream_t s) {
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec)) return e;
   WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12 (got %d)", G);
@@ -167,6 +195,8 @@ static int apply_cb(void* res_cb, const void* psi_cb, const void* gauge_cb, cons
   WD_REQUIRE(tcb == 0 || tcb == 1, "checkerboard must be 0 or 1 (got %d)", tcb);
   WD_REQUIRE(res_cb && psi_cb && gauge_cb, "NULL field pointer");
   WD_REQUIRE(res_cb != psi_cb, "res and psi must be different checkerboard buffers");
+  WD_REQUIRE(!xp_cb || nd == 4, "the fused x + a*D psi epilogue is built for the 4D operator only");
+  WD_REQUIRE(xp_cb != psi_cb, "x and psi must be different checkerboard buffers");
   ApplyArgs a;
   a.g = make_geom(latt, nd);
   const size_t planes = (size_t)spinor_vec(prec) * 16 * a.g.vh;
@@ -175,14 +205,24 @@ static int apply_cb(void* res_cb, const void* psi_cb, const void* gauge_cb, cons
   a.res = (v4u*)res_cb;
   a.res_norm = (float*)((char*)res_cb + planes);
   a.gauge = (const v4u*)gauge_cb;
+  a.xp = (const v4u*)xp_cb;
+  a.xp_norm = xp_cb ? (const float*)((const char*)xp_cb + planes) : nullptr;
+  a.a = xa;
   a.tcb = tcb;
-  a.remap = g_remap;
+  const LaunchCfg lc = launch_cfg(nd, prec, G);
+  a.remap = lc.remap;
+  a.block = lc.block;
   if (nd == 4)
     launch_dslash_nd<4>(a, prec, G, isign, s);
   else
     launch_dslash_nd<3>(a, prec, G, isign, s);
   WD_LAUNCHED();
   return 0;
+}
+
+static int apply_cb(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd,
+                    int prec, int G, int isign, int tcb, hipStream_t s) {
+  return apply_cb_ex(res_cb, psi_cb, gauge_cb, latt, nd, prec, G, isign, tcb, nullptr, 0.0, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,18 +416,32 @@ size_t dslash_gauge_cb_bytes(const int latt[4], int nd, int prec, int G) {
   return gauge_cb_bytes(make_geom(latt, nd).vh, nd, prec, G);
 }
 
-int dslash_set_launch(int block_threads, int xcd_remap) {
+int dslash_set_launch(int nd, int prec, int gauge_reals, int block_threads, int xcd_remap) {
   clear_error();
   WD_REQUIRE(block_threads >= 64 && block_threads <= 256 && block_threads % 64 == 0,
              "block_threads must be 64, 128, 192 or 256 (got %d)", block_threads);
-  g_block = block_threads;
-  g_remap = xcd_remap ? 1 : 0;
+  LaunchCfg c{block_threads, xcd_remap ? 1 : 0};
+  if (nd == 0) {
+    for (int n : {3, 4})
+      for (int p = 0; p < 4; ++p)
+        for (int G : {18, 12}) set_launch_cfg(n, p, G, c);
+    return 0;
+  }
+  WD_REQUIRE(nd == 3 || nd == 4, "nd must be 0 (all), 3 or 4");
+  if (int e = check_prec(prec)) return e;
+  WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
+  set_launch_cfg(nd, prec, gauge_reals, c);
   return 0;
 }
 
-int dslash_get_launch(int* block_threads, int* xcd_remap) {
-  if (block_threads) *block_threads = g_block;
-  if (xcd_remap) *xcd_remap = g_remap;
+int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int* xcd_remap) {
+  clear_error();
+  WD_REQUIRE(nd == 3 || nd == 4, "nd must be 3 or 4");
+  if (int e = check_prec(prec)) return e;
+  WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
+  LaunchCfg c = launch_cfg(nd, prec, gauge_reals);
+  if (block_threads) *block_threads = c.block;
+  if (xcd_remap) *xcd_remap = c.remap;
   return 0;
 }
 
@@ -396,6 +450,14 @@ int dslash_apply(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb, c
   clear_error();
   return apply_cb(d_res_cb, d_psi_cb, d_gauge_cb, latt, nd, prec, gauge_reals, isign, target_cb,
                   (hipStream_t)stream);
+}
+
+int dslash_apply_xpay(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb, const void* d_x_cb, double a,
+                      const int latt[4], int prec, int gauge_reals, int isign, int target_cb, void* stream) {
+  clear_error();
+  WD_REQUIRE(d_x_cb, "x is NULL");
+  return apply_cb_ex(d_res_cb, d_psi_cb, d_gauge_cb, latt, 4, prec, gauge_reals, isign, target_cb, d_x_cb, a,
+                     (hipStream_t)stream);
 }
 
 int dslash_apply_full(void* d_res, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec,

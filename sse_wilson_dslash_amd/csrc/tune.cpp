@@ -1,0 +1,202 @@
+// Launch-shape autotuner and the batched (multi-RHS / multi-lattice) entry point.
+//
+// The tuner times every candidate shape with HIP events on the caller's
+// stream, on the caller's own fields, and keeps the fastest per kernel family:
+//   single-RHS Dslash:  (nd, format, link reals) -> threads per block, XCD remap
+//   batched Dslash:     (format, link reals, persistent) -> RHS block, threads
+//                        per block, persistent grid size
+// Choices can be saved to and reloaded from a small text file, so a run tunes
+// once per machine and later processes start tuned.
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include "dslash_common.h"
+#include "dslash_internal.h"
+#include "dslash_mi355x.h"
+#include "mrhs.h"
+#include "status.h"
+
+namespace wd {
+
+struct MrhsCfg {
+  int R;
+  int block;
+  int grid;  // persistent grid blocks (0 for the one-tile-per-block launch)
+};
+
+// [format F32/F64][G==12][persistent]
+static MrhsCfg g_mrhs[2][2][2] = {{{{2, 128, 0}, {2, 128, 2048}}, {{2, 128, 0}, {2, 128, 2048}}},
+                                  {{{2, 128, 0}, {2, 128, 2048}}, {{2, 128, 0}, {2, 128, 2048}}}};
+
+static MrhsCfg& mrhs_entry(int prec, int G, int persistent) {
+  return g_mrhs[prec == F64][G == 12][persistent ? 1 : 0];
+}
+
+static int mrhs_run(const dslash_mrhs_problem* probs, int nprob, int prec, int G, int R, int block, int grid,
+                    hipStream_t s) {
+  MrhsList list;
+  if (int e = mrhs_build_list(probs, nprob, prec, R, block, &list)) return e;
+  uint32_t g = grid > 0 ? (uint32_t)grid : list.total_tiles;
+  return mrhs_launch(list, prec, G, R, block, g, s);
+}
+
+template <typename F>
+static int time_it(hipStream_t s, int reps, F launch, float* ms) {
+  hipEvent_t e0, e1;
+  WD_HIP(hipEventCreate(&e0));
+  WD_HIP(hipEventCreate(&e1));
+  int err = launch();  // warm-up (also validates the shape)
+  if (!err) {
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < reps && !err; ++i) err = launch();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return err;
+}
+
+}  // namespace wd
+
+using namespace wd;
+
+extern "C" {
+
+int dslash_apply_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int G, int rhs_block, int block_threads,
+                      int grid_blocks, void* stream) {
+  clear_error();
+  WD_REQUIRE(prec == F32 || prec == F64, "multi-RHS Dslash is built for F32 and F64");
+  WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
+  WD_REQUIRE(grid_blocks >= 0, "grid_blocks must be >= 0");
+  const MrhsCfg& c = mrhs_entry(prec, G, grid_blocks > 0);
+  int R = rhs_block ? rhs_block : c.R;
+  int block = block_threads ? block_threads : c.block;
+  return mrhs_run(probs, nprob, prec, G, R, block, grid_blocks, (hipStream_t)stream);
+}
+
+int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec, int G,
+                      void* stream, int* block_threads, int* xcd_remap, float* best_ms) {
+  clear_error();
+  if (int e = dslash_check_lattice(latt, nd)) return e;
+  WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
+  WD_REQUIRE(prec >= F32 && prec <= BF16, "unknown precision");
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t vh = (uint32_t)((uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2);
+  const int reps = vh > (1u << 18) ? 20 : 200;
+  const LaunchCfg saved = launch_cfg(nd, prec, G);
+  LaunchCfg best = saved;
+  float best_t = 1e30f;
+  for (int block : {64, 128, 192, 256})
+    for (int remap : {0, 1}) {
+      set_launch_cfg(nd, prec, G, LaunchCfg{block, remap});
+      float t = 0;
+      int e = time_it(s, reps, [&]() { return dslash_apply_full(d_res, d_psi, d_gauge, latt, nd, prec, G, 1, 0, stream); },
+                      &t);
+      if (e) {
+        set_launch_cfg(nd, prec, G, saved);
+        return e;
+      }
+      if (t < best_t) {
+        best_t = t;
+        best = LaunchCfg{block, remap};
+      }
+    }
+  set_launch_cfg(nd, prec, G, best);
+  if (block_threads) *block_threads = best.block;
+  if (xcd_remap) *xcd_remap = best.remap;
+  if (best_ms) *best_ms = best_t;
+  return 0;
+}
+
+int dslash_tune_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int G, int persistent, void* stream,
+                     int* rhs_block, int* block_threads, int* grid_blocks, float* best_ms) {
+  clear_error();
+  WD_REQUIRE(prec == F32 || prec == F64, "multi-RHS Dslash is built for F32 and F64");
+  WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
+  hipStream_t s = (hipStream_t)stream;
+  MrhsCfg best{0, 0, 0};
+  float best_t = 1e30f;
+  const int grids_p[] = {512, 1024, 2048, 4096};
+  const int grids_np[] = {0};
+  const int* grids = persistent ? grids_p : grids_np;
+  const int ng = persistent ? 4 : 1;
+  for (int R : {1, 2, 4})
+    for (int block : {64, 128, 256})
+      for (int gi = 0; gi < ng; ++gi) {
+        float t = 0;
+        int grid = grids[gi];
+        int e = time_it(s, 10, [&]() { return mrhs_run(probs, nprob, prec, G, R, block, grid, s); }, &t);
+        if (e) return e;
+        if (t < best_t) {
+          best_t = t;
+          best = MrhsCfg{R, block, grid};
+        }
+      }
+  mrhs_entry(prec, G, persistent) = best;
+  if (rhs_block) *rhs_block = best.R;
+  if (block_threads) *block_threads = best.block;
+  if (grid_blocks) *grid_blocks = best.grid;
+  if (best_ms) *best_ms = best_t;
+  return 0;
+}
+
+int dslash_tune_save(const char* path) {
+  clear_error();
+  WD_REQUIRE(path, "path is NULL");
+  FILE* f = fopen(path, "w");
+  WD_REQUIRE(f, "cannot write %s", path);
+  fprintf(f, "# dslash_mi355x launch shapes\n");
+  for (int nd : {3, 4})
+    for (int p = 0; p < 4; ++p)
+      for (int G : {18, 12}) {
+        LaunchCfg c = launch_cfg(nd, p, G);
+        fprintf(f, "apply %d %d %d %d %d\n", nd, p, G, c.block, c.remap);
+      }
+  for (int p : {F32, F64})
+    for (int G : {18, 12})
+      for (int per : {0, 1}) {
+        MrhsCfg c = mrhs_entry(p, G, per);
+        fprintf(f, "mrhs %d %d %d %d %d %d\n", p, G, per, c.R, c.block, c.grid);
+      }
+  fclose(f);
+  return 0;
+}
+
+int dslash_tune_load(const char* path) {
+  clear_error();
+  WD_REQUIRE(path, "path is NULL");
+  FILE* f = fopen(path, "r");
+  WD_REQUIRE(f, "cannot read %s", path);
+  char line[256];
+  int bad = 0;
+  while (fgets(line, sizeof(line), f)) {
+    int a[6];
+    if (line[0] == '#') continue;
+    if (sscanf(line, "apply %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4]) == 5) {
+      if ((a[0] == 3 || a[0] == 4) && a[1] >= 0 && a[1] < 4 && (a[2] == 18 || a[2] == 12) && a[3] >= 64 &&
+          a[3] <= 256 && a[3] % 64 == 0)
+        set_launch_cfg(a[0], a[1], a[2], LaunchCfg{a[3], a[4] ? 1 : 0});
+      else
+        ++bad;
+    } else if (sscanf(line, "mrhs %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]) == 6) {
+      if ((a[0] == F32 || a[0] == F64) && (a[1] == 18 || a[1] == 12) && (a[3] == 1 || a[3] == 2 || a[3] == 4) &&
+          a[4] >= 64 && a[4] <= 256 && a[4] % 64 == 0 && a[5] >= 0)
+        mrhs_entry(a[0], a[1], a[2]) = MrhsCfg{a[3], a[4], a[2] ? (a[5] > 0 ? a[5] : 2048) : 0};
+      else
+        ++bad;
+    } else if (line[0] != '\n') {
+      ++bad;
+    }
+  }
+  fclose(f);
+  WD_REQUIRE(bad == 0, "%d malformed line(s) in %s", bad, path);
+  return 0;
+}
+
+}  // extern "C"

@@ -137,6 +137,29 @@ class WilsonDslash:
                                                 None, 0, current_stream()))
         return d.cpu().numpy().reshape(self.V, 4, 3, 2)
 
+    def pack_cb(self, aos, cb: int, prec: int | None = None) -> torch.Tensor:
+        """One checkerboard of a legacy full-lattice spinor -> a single-cb device field."""
+        p = self.prec if prec is None else prec
+        if isinstance(aos, np.ndarray):
+            pin = _prec_of_dtype(aos.dtype)
+            d = torch.from_numpy(np.ascontiguousarray(aos).reshape(-1)).to(self.device)
+        else:
+            pin = _prec_of_dtype(aos.dtype)
+            d = aos.reshape(-1).to(self.device).contiguous()
+        out = self.new_spinor(False, p)
+        check(self.lib.dslash_pack_spinor(ptr(out), int(cb), ptr(d), pin, self._L, self.nd, p, None,
+                                          current_stream()))
+        return out
+
+    def unpack_cb(self, dev_cb: torch.Tensor, cb: int, dtype=np.float64, prec: int | None = None) -> np.ndarray:
+        """A single-cb device field -> legacy full-lattice numpy array (other cb zero)."""
+        p = self.prec if prec is None else prec
+        pout = _prec_of_dtype(np.dtype(dtype))
+        d = torch.zeros(self.V * 24, dtype=_TORCH[pout], device=self.device)
+        check(self.lib.dslash_unpack_spinor(ptr(d), int(cb), ptr(dev_cb), p, self._L, self.nd, pout, None, 0,
+                                            current_stream()))
+        return d.cpu().numpy().reshape(self.V, 4, 3, 2)
+
     def gen_spinor(self, seed: int = 1, field: int = 0, prec: int | None = None) -> torch.Tensor:
         """Seeded Gaussian spinor generated in HBM, returned in device layout."""
         pin = F64 if (self.prec if prec is None else prec) == F64 else F32

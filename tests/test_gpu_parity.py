@@ -148,10 +148,13 @@ def test_launch_shapes_agree(oracle, block, remap):
     latt = (8, 6, 4, 10)
     u = gen_gauge_host(latt, 4, seed=2, dtype=np.float32)
     psi = gen_spinor_host(latt, 4, seed=2, dtype=np.float32)
-    _lib.check(_lib.core().dslash_set_launch(block, remap))
+    import ctypes as C
+    b, r = C.c_int(), C.c_int()
+    _lib.check(_lib.core().dslash_get_launch(4, F32, 18, C.byref(b), C.byref(r)))
+    _lib.check(_lib.core().dslash_set_launch(4, F32, 18, block, remap))
     try:
         got = run_gpu(u, psi, latt, 4, F32, 18, 1, 0)
     finally:
-        _lib.check(_lib.core().dslash_set_launch(256, 1))
+        _lib.check(_lib.core().dslash_set_launch(4, F32, 18, b.value, r.value))
     want = oracle.dslash(u, psi, np.zeros_like(psi), latt, 1, 0)
     check(got, want, latt, 0, F32)

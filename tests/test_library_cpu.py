@@ -140,12 +140,35 @@ def test_sizes_and_validation():
     assert st == _lib.ERR_INVALID
     st = lib.dslash_apply(1, 2, 3, _lib.latt_arg((4, 4, 4, 4)), 4, _lib.F32, 10, 1, 0, None)
     assert st == _lib.ERR_INVALID
-    assert lib.dslash_set_launch(100, 1) == _lib.ERR_INVALID
-    assert lib.dslash_set_launch(128, 0) == 0
+    assert lib.dslash_set_launch(0, 0, 0, 100, 1) == _lib.ERR_INVALID
     b, r = C.c_int(), C.c_int()
-    lib.dslash_get_launch(C.byref(b), C.byref(r))
+    assert lib.dslash_get_launch(4, _lib.F32, 18, C.byref(b), C.byref(r)) == 0
+    saved = (b.value, r.value)
+    assert lib.dslash_set_launch(4, _lib.F32, 18, 128, 0) == 0
+    lib.dslash_get_launch(4, _lib.F32, 18, C.byref(b), C.byref(r))
     assert (b.value, r.value) == (128, 0)
-    assert lib.dslash_set_launch(256, 1) == 0
+    lib.dslash_get_launch(3, _lib.F32, 18, C.byref(b), C.byref(r))
+    assert b.value in (64, 128, 192, 256)
+    assert lib.dslash_set_launch(4, _lib.F32, 18, *saved) == 0
+
+
+def test_tune_file_round_trip(tmp_path):
+    lib = _lib.core()
+    b, r = C.c_int(), C.c_int()
+    lib.dslash_get_launch(3, _lib.F64, 12, C.byref(b), C.byref(r))
+    saved = (b.value, r.value)
+    assert lib.dslash_set_launch(3, _lib.F64, 12, 192, 0) == 0
+    path = str(tmp_path / "tune.txt").encode()
+    assert lib.dslash_tune_save(path) == 0
+    assert lib.dslash_set_launch(3, _lib.F64, 12, 64, 1) == 0
+    assert lib.dslash_tune_load(path) == 0
+    lib.dslash_get_launch(3, _lib.F64, 12, C.byref(b), C.byref(r))
+    assert (b.value, r.value) == (192, 0)
+    assert lib.dslash_set_launch(3, _lib.F64, 12, *saved) == 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text("apply 4 0 18 100 1\n")
+    assert lib.dslash_tune_load(str(bad).encode()) == _lib.ERR_INVALID
+    assert lib.dslash_tune_load(b"/nonexistent/tune.txt") == _lib.ERR_INVALID
 
 
 @pytest.mark.parametrize("prec", [32, 64])

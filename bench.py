@@ -111,6 +111,115 @@ def cpu_leg(seconds: float, threads: int):
                       f"{med * 1e3:.1f} ms/call, OpenMP over target sites"}
 
 
+def copy_leg(gib: float = 2.0, reps: int = 20):
+    """Device-to-device copy bandwidth (read + write bytes / time): the practical HBM ceiling."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 2 * n * 4 / (ms * 1e-3) / 1e9
+    log(f"[copy] {gib:.1f} GiB device copy: {ms:.3f} ms, {gbs:.0f} GB/s (read+write)")
+    del a, b
+    torch.cuda.empty_cache()
+    return {"gbs": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes": 2 * n * 4}
+
+
+def cg_leg(latt=(48, 48, 48, 96), kappa=0.12, tol=1e-10):
+    """Mixed-precision (F16 spinors + norms, 12-real F16 links, FP64 reliable
+    updates) vs plain FP64 even-odd CG on Mhat^dag Mhat, same Gaussian source,
+    time to FP64 true residual tol."""
+    import torch
+    from sse_wilson_dslash_amd import F16, F64
+    from sse_wilson_dslash_amd._lib import check, latt_arg, ptr
+    from sse_wilson_dslash_amd.device import WilsonDslash, current_stream
+    from sse_wilson_dslash_amd.solver import WilsonCG
+
+    op = WilsonDslash(latt, 4, F64, 18)
+    V = op.V
+    d_u = torch.empty(V * 72, dtype=torch.float64, device="cuda")
+    check(op.lib.dslash_gen_gauge_device(ptr(d_u), F64, latt_arg(latt), 4, 1, current_stream()))
+    src = op.gen_spinor(seed=1, field=7)
+    b = op.cb_view(src, 0).clone()
+    del src
+    out = {}
+    for name, prec, G in (("fp64", F64, 18), ("mixed_f16", F16, 12)):
+        cg = WilsonCG(latt, kappa=kappa, tol=tol, prec_sloppy=prec, gauge_reals_sloppy=G, delta=0.1)
+        cg.load_gauge(d_u)
+        x = op.new_spinor(full=False)
+        cg.solve(x, b)  # warm-up (graph capture, first-touch)
+        r = cg.solve(x, b)
+        log(f"[cg] {name}: {r['seconds'] * 1e3:.1f} ms, {r['iterations']} iterations, "
+            f"{r['reliable_updates']} reliable updates, true residual {r['true_residual']:.3e}")
+        out[name] = {"ms": round(r["seconds"] * 1e3, 2), "iterations": r["iterations"],
+                     "reliable_updates": r["reliable_updates"], "true_residual": r["true_residual"],
+                     "converged": r["converged"]}
+        cg.close()
+        del x
+        torch.cuda.empty_cache()
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed_f16"]["ms"], 3)
+    out["config"] = {"lattice": list(latt), "kappa": kappa, "tol": tol, "operator": "Mhat^dag Mhat, even-odd",
+                     "source": "Gaussian, even checkerboard, seed 1"}
+    log(f"[cg] mixed vs FP64 speedup: {out['speedup_mixed_vs_fp64']:.2f}x")
+    del op, d_u, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def mrhs_leg(reps: int = 50):
+    """32 right-hand sides on one 24^3x48 gauge, and one persistent launch over
+    8^4, 16^3x32, 24^3x48 with 4 RHS each (FP32, 18-real links), tuned shapes."""
+    import torch
+    from sse_wilson_dslash_amd import F32
+    from sse_wilson_dslash_amd.batch import (BatchItem, RhsBatch, apply_batch, compulsory_bytes, flops,
+                                            tune_batch)
+    from sse_wilson_dslash_amd.device import WilsonDslash
+
+    def item(latt, nrhs, seed):
+        op = WilsonDslash(latt, 4, F32, 18)
+        op.gen_gauge(seed=seed)
+        psi = RhsBatch(op, nrhs)
+        for k in range(nrhs):
+            psi.field(k).copy_(op.cb_view(op.gen_spinor(seed=seed, field=k), 1))
+        return BatchItem(op, psi, RhsBatch(op, nrhs, zero=True), 1, 0)
+
+    def timed(items, grid):
+        apply_batch(items, grid_blocks=grid)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            apply_batch(items, grid_blocks=grid)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    items = [item((24, 24, 24, 48), 32, 1)]
+    t = tune_batch(items, persistent=False)
+    ms = timed(items, 0)
+    out["rhs32_24c48"] = {"ms": round(ms, 4), "gflops": round(flops(items) / (ms * 1e-3) / 1e9, 1),
+                          "hbm_gbs": round(compulsory_bytes(items) / (ms * 1e-3) / 1e9, 1), "tuned": t}
+    log(f"[mrhs] 32 RHS 24^3x48: {ms * 1e3:.1f} us/launch, {out['rhs32_24c48']['gflops']:.0f} GFLOP/s, tuned {t}")
+    del items
+    items = [item((8, 8, 8, 8), 4, 2), item((16, 16, 16, 32), 4, 3), item((24, 24, 24, 48), 4, 4)]
+    t = tune_batch(items, persistent=True)
+    ms = timed(items, t["grid_blocks"])
+    out["persistent_uneven"] = {"ms": round(ms, 4), "gflops": round(flops(items) / (ms * 1e-3) / 1e9, 1),
+                                "hbm_gbs": round(compulsory_bytes(items) / (ms * 1e-3) / 1e9, 1), "tuned": t}
+    log(f"[mrhs] persistent 8^4+16^3x32+24^3x48 x4 RHS: {ms * 1e3:.1f} us/launch, "
+        f"{out['persistent_uneven']['gflops']:.0f} GFLOP/s, tuned {t}")
+    del items
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,7 +228,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 3D / 12-real variants")
+    ap.add_argument("--no-cg", action="store_true", help="skip the 48^3x96 mixed vs FP64 CG")
+    ap.add_argument("--no-mrhs", action="store_true", help="skip the batched Dslash legs")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="only a 2 GiB calibration copy and the headline kernel (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -133,8 +246,12 @@ def main() -> int:
         return 2
 
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    if args.pmc_workload:
+        copy_leg(2.0, 3)
+        gpu_leg(args.steps, args.warmup, "f32", 4, 18)
+        return 0
     head = gpu_leg(args.steps, args.warmup, "f32", 4, 18)
-    extra = {}
+    extra = {"copy": copy_leg()}
     if not args.no_extra:
         n = max(200, args.steps // 4)
         for key, (p, nd, G) in {"f64_4d": ("f64", 4, 18), "f32_4d_g12": ("f32", 4, 12),
@@ -142,6 +259,10 @@ def main() -> int:
             r = gpu_leg(n, 20, p, nd, G)
             extra[key] = {"gflops": round(r["gflops"], 1), "us_per_call": round(r["ms"] * 1e3, 2),
                           "hbm_gbs": round(r["gbs"], 1), "hbm_frac": round(r["gbs"] / PEAK_HBM_GBS, 4)}
+    if not args.no_mrhs:
+        extra["mrhs"] = mrhs_leg()
+    if not args.no_cg:
+        extra["cg_48c96"] = cg_leg()
 
     cpu = None
     if not args.no_cpu:

@@ -56,7 +56,7 @@ class WilsonDslash:
         self.gb = int(self.lib.dslash_gauge_cb_bytes(self._L, self.nd, self.prec, self.G))
         if self.sb == 0 or self.gb == 0:
             check(self.lib.dslash_last_error() or _lib.ERR_INVALID)
-        self.gauge = torch.empty(2 * self.gb, dtype=torch.uint8, device=self.device)
+        self.gauge = None  # allocated by load_gauge
         self.has_gauge = False
 
     # ---- sizes and allocation ------------------------------------------
@@ -88,6 +88,8 @@ class WilsonDslash:
         d_tab = None
         if site_table is not None:
             d_tab = torch.as_tensor(np.asarray(site_table, dtype=np.int32)).to(self.device)
+        if self.gauge is None:
+            self.gauge = torch.empty(2 * self.gb, dtype=torch.uint8, device=self.device)
         for t in (0, 1):
             check(self.lib.dslash_pack_gauge(ptr(self.gauge) + t * self.gb, t, ptr(d_u), pin, self._L, self.nd,
                                              self.prec, self.G, ptr(d_tab), stream))

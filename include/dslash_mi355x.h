@@ -174,6 +174,8 @@ typedef struct {
   int gauge_reals_sloppy; /* 18 or 12 */
   int use_graph;          /* replay the inner iteration as a captured hipGraph */
   int zero_guess;         /* start from x = 0 instead of the contents of x */
+  int gauge_reals_precise;/* links of the FP64 operator (reliable updates, true
+                             residual): 18 (or 0) = full matrices, 12 = two rows */
 } dslash_cg_params;
 
 typedef struct {

@@ -32,7 +32,7 @@ class CgParams(C.Structure):
     """dslash_cg_params"""
     _fields_ = [("kappa", C.c_double), ("tol", C.c_double), ("delta", C.c_double), ("max_iter", C.c_int),
                 ("prec_sloppy", C.c_int), ("gauge_reals_sloppy", C.c_int), ("use_graph", C.c_int),
-                ("zero_guess", C.c_int)]
+                ("zero_guess", C.c_int), ("gauge_reals_precise", C.c_int)]
 
 
 class CgResult(C.Structure):

@@ -64,5 +64,7 @@ int blas_residual(Vec out, Vec b, Vec Ay, uint32_t vh, double* partial, double* 
 int blas_axpby(double a, Vec x, double b, Vec y, uint32_t vh, hipStream_t s);
 // finalize only (FIN_BETA / FIN_COPY_RR)
 int blas_scalar_op(double* scal, int op, hipStream_t s);
+// sum nparts (re, im) partials written by another kernel, then apply op
+int blas_finalize(double* partial, int nparts, double* scal, int op, hipStream_t s);
 
 }  // namespace wd

@@ -40,9 +40,10 @@ struct CgSolver {
   int ps;   // sloppy format
   int Gs;   // sloppy link reals
   int px;   // solution accumulator format
+  int Gh;   // link reals of the FP64 operator
   size_t g64_cb = 0, gs_cb = 0;
-  void* g64 = nullptr;  // FP64 18-real bundles, both target cbs
-  void* gs = nullptr;   // sloppy bundles (== g64 for an FP64/18 inner iteration)
+  void* g64 = nullptr;  // FP64 bundles (Gh reals), both target cbs
+  void* gs = nullptr;   // sloppy bundles (== g64 for an FP64 inner iteration with Gs == Gh)
   bool gs_owned = false;
   // FP64 work vectors (even cb unless noted)
   void *r_hi = nullptr, *q_hi = nullptr, *Ay_hi = nullptr, *t_hi = nullptr /* odd */;
@@ -80,15 +81,16 @@ static void cg_free(CgSolver* s) {
 }
 
 // out = A in = Mhat^dagger Mhat in  (even cb), in format prec with links g.
+// With qnorm, the per-block partials of |q|^2 = |Mhat in|^2 = <in, A in> are written there.
 static int apply_normal(const CgSolver* s, void* out, const void* in, void* t_odd, void* q_even, int prec, int G,
-                        const void* g, size_t gcb) {
+                        const void* g, size_t gcb, double* qnorm = nullptr) {
   const double k2 = -s->prm.kappa * s->prm.kappa;
   const char* g0 = (const char*)g;
   const char* g1 = g0 + gcb;
   hipStream_t st = s->stream;
   // q = in - kappa^2 D_eo D_oe in
   if (int e = apply_cb_ex(t_odd, in, g1, s->latt, 4, prec, G, +1, 1, nullptr, 0.0, st)) return e;
-  if (int e = apply_cb_ex(q_even, t_odd, g0, s->latt, 4, prec, G, +1, 0, in, k2, st)) return e;
+  if (int e = apply_cb_ex(q_even, t_odd, g0, s->latt, 4, prec, G, +1, 0, in, k2, st, qnorm)) return e;
   // out = q - kappa^2 (D^dag)_eo (D^dag)_oe q
   if (int e = apply_cb_ex(t_odd, q_even, g1, s->latt, 4, prec, G, -1, 1, nullptr, 0.0, st)) return e;
   if (int e = apply_cb_ex(out, t_odd, g0, s->latt, 4, prec, G, -1, 0, q_even, k2, st)) return e;
@@ -96,9 +98,11 @@ static int apply_normal(const CgSolver* s, void* out, const void* in, void* t_od
 }
 
 // One inner iteration: Ap = A p ; alpha = rr / <p,Ap> ; x += alpha p ; r -= alpha Ap ; RRNEW = |r|^2
+// <p, A p> = |Mhat p|^2 comes from the epilogue of the Dslash that forms Mhat p.
 static int sloppy_iteration(CgSolver* s) {
-  if (int e = apply_normal(s, s->Ap, s->p, s->t, s->q, s->ps, s->Gs, s->gs, s->gs_cb)) return e;
-  if (int e = blas_dot({s->p, s->ps}, {s->Ap, s->ps}, s->vh, s->partial, s->scal, FIN_ALPHA, s->stream)) return e;
+  if (int e = apply_normal(s, s->Ap, s->p, s->t, s->q, s->ps, s->Gs, s->gs, s->gs_cb, s->partial)) return e;
+  if (int e = blas_finalize(s->partial, apply_blocks(s->latt, 4, s->ps, s->Gs), s->scal, FIN_ALPHA, s->stream))
+    return e;
   return blas_cg_update({s->xlo, s->px}, {s->p, s->ps}, {s->r, s->ps}, {s->Ap, s->ps}, s->vh, s->partial, s->scal,
                         s->stream);
 }
@@ -148,7 +152,9 @@ void* dslash_cg_create(const int latt[4], const dslash_cg_params* prm) {
   }
   if (dslash_check_lattice(latt, 4)) return nullptr;
   if (!(prm->prec_sloppy == F16 || prm->prec_sloppy == BF16 || prm->prec_sloppy == F32 || prm->prec_sloppy == F64) ||
-      !(prm->gauge_reals_sloppy == 18 || prm->gauge_reals_sloppy == 12) || !(prm->tol > 0) || prm->max_iter <= 0) {
+      !(prm->gauge_reals_sloppy == 18 || prm->gauge_reals_sloppy == 12) ||
+      !(prm->gauge_reals_precise == 0 || prm->gauge_reals_precise == 18 || prm->gauge_reals_precise == 12) ||
+      !(prm->tol > 0) || prm->max_iter <= 0) {
     set_error(DSLASH_ERR_INVALID, "invalid CG parameters (prec %d, gauge_reals %d, tol %g, max_iter %d)",
               prm->prec_sloppy, prm->gauge_reals_sloppy, prm->tol, prm->max_iter);
     return nullptr;
@@ -165,17 +171,19 @@ void* dslash_cg_create(const int latt[4], const dslash_cg_params* prm) {
   s->Gs = prm->gauge_reals_sloppy;
   s->px = s->ps == F64 ? F64 : F32;
   const size_t b64 = spinor_cb_bytes(s->vh, F64), bs = spinor_cb_bytes(s->vh, s->ps), bx = spinor_cb_bytes(s->vh, s->px);
-  s->g64_cb = gauge_cb_bytes(s->vh, 4, F64, 18);
+  s->Gh = prm->gauge_reals_precise == 12 ? 12 : 18;
+  s->g64_cb = gauge_cb_bytes(s->vh, 4, F64, s->Gh);
+  const size_t nparts = (size_t)((s->vh + 63) / 64) > (size_t)RED_BLOCKS ? (size_t)((s->vh + 63) / 64) : (size_t)RED_BLOCKS;
   s->gs_cb = gauge_cb_bytes(s->vh, 4, s->ps, s->Gs);
   bool ok = hipStreamCreate(&s->stream) == hipSuccess && !cg_alloc(&s->g64, 2 * s->g64_cb) &&
             !cg_alloc(&s->r_hi, b64) && !cg_alloc(&s->q_hi, b64) && !cg_alloc(&s->Ay_hi, b64) &&
             !cg_alloc(&s->t_hi, b64) && !cg_alloc(&s->r, bs) && !cg_alloc(&s->p, bs) && !cg_alloc(&s->Ap, bs) &&
             !cg_alloc(&s->q, bs) && !cg_alloc(&s->t, bs) && !cg_alloc(&s->xlo, bx) &&
             !cg_alloc((void**)&s->scal, S_COUNT * sizeof(double)) &&
-            !cg_alloc((void**)&s->partial, 2 * RED_BLOCKS * sizeof(double)) &&
+            !cg_alloc((void**)&s->partial, 2 * nparts * sizeof(double)) &&
             hipHostMalloc((void**)&s->h_scal, S_COUNT * sizeof(double), 0) == hipSuccess;
   if (ok) {
-    if (s->ps == F64 && s->Gs == 18) {
+    if (s->ps == F64 && s->Gs == s->Gh) {
       s->gs = s->g64;
     } else {
       ok = !cg_alloc(&s->gs, 2 * s->gs_cb);
@@ -194,8 +202,9 @@ int dslash_cg_set_params(void* solver, const dslash_cg_params* prm) {
   clear_error();
   CgSolver* s = (CgSolver*)solver;
   WD_REQUIRE(s && prm, "NULL argument");
-  WD_REQUIRE(prm->prec_sloppy == s->prm.prec_sloppy && prm->gauge_reals_sloppy == s->prm.gauge_reals_sloppy,
-             "the sloppy format is fixed at dslash_cg_create");
+  WD_REQUIRE(prm->prec_sloppy == s->prm.prec_sloppy && prm->gauge_reals_sloppy == s->prm.gauge_reals_sloppy &&
+                 prm->gauge_reals_precise == s->prm.gauge_reals_precise,
+             "the link and spinor formats are fixed at dslash_cg_create");
   WD_REQUIRE(prm->tol > 0 && prm->max_iter > 0, "invalid tol / max_iter");
   if (prm->kappa != s->prm.kappa) {  // kappa is baked into the captured graphs
     WD_HIP(hipStreamSynchronize(s->stream));
@@ -223,7 +232,7 @@ int dslash_cg_load_gauge(void* solver, const void* d_u_packed, int prec_in, void
   // order the packing after the caller's stream
   if (stream) WD_HIP(hipStreamSynchronize((hipStream_t)stream));
   for (int t = 0; t < 2; ++t) {
-    if (int e = dslash_pack_gauge((char*)s->g64 + t * s->g64_cb, t, d_u_packed, prec_in, s->latt, 4, F64, 18, nullptr,
+    if (int e = dslash_pack_gauge((char*)s->g64 + t * s->g64_cb, t, d_u_packed, prec_in, s->latt, 4, F64, s->Gh, nullptr,
                                   s->stream))
       return e;
     if (s->gs_owned)
@@ -242,7 +251,7 @@ int dslash_cg_apply_op(void* solver, void* d_out, const void* d_in, int sloppy) 
   WD_REQUIRE(s && d_out && d_in && d_out != d_in, "bad arguments");
   WD_REQUIRE(s->have_gauge, "dslash_cg_load_gauge first");
   int e = sloppy ? apply_normal(s, d_out, d_in, s->t, s->q, s->ps, s->Gs, s->gs, s->gs_cb)
-                 : apply_normal(s, d_out, d_in, s->t_hi, s->q_hi, F64, 18, s->g64, s->g64_cb);
+                 : apply_normal(s, d_out, d_in, s->t_hi, s->q_hi, F64, s->Gh, s->g64, s->g64_cb);
   if (e) return e;
   WD_HIP(hipStreamSynchronize(s->stream));
   return 0;
@@ -273,7 +282,7 @@ int dslash_cg_solve(void* solver, void* d_x, const void* d_b, dslash_cg_result* 
     if (int e = blas_copy(RH, B, vh, st)) return e;
     if (int e = blas_norm2(RH, vh, s->partial, s->scal, FIN_SET_RR, st)) return e;
   } else {
-    if (int e = apply_normal(s, s->Ay_hi, d_x, s->t_hi, s->q_hi, F64, 18, s->g64, s->g64_cb)) return e;
+    if (int e = apply_normal(s, s->Ay_hi, d_x, s->t_hi, s->q_hi, F64, s->Gh, s->g64, s->g64_cb)) return e;
     if (int e = blas_residual(RH, B, AY, vh, s->partial, s->scal, FIN_SET_RR, st)) return e;
   }
   if (int e = blas_copy(R, RH, vh, st)) return e;
@@ -305,7 +314,7 @@ int dslash_cg_solve(void* solver, void* d_x, const void* d_b, dslash_cg_result* 
     if (reliable) {
       // fold the low-precision solution in, recompute the true residual in FP64
       if (int e = blas_accumulate(X, XL, vh, st)) return e;
-      if (int e = apply_normal(s, s->Ay_hi, d_x, s->t_hi, s->q_hi, F64, 18, s->g64, s->g64_cb)) return e;
+      if (int e = apply_normal(s, s->Ay_hi, d_x, s->t_hi, s->q_hi, F64, s->Gh, s->g64, s->g64_cb)) return e;
       if (int e = blas_residual(RH, B, AY, vh, s->partial, s->scal, FIN_RRNEW, st)) return e;
       if (int e = read_scalars(s)) return e;
       ++updates;

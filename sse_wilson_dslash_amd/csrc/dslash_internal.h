@@ -16,7 +16,10 @@ void set_launch_cfg(int nd, int prec, int G, LaunchCfg c);
 
 // res_cb = D psi_cb (xp_cb == nullptr) or res_cb = xp_cb + a * D psi_cb, all on
 // one checkerboard in the device layout of `prec`; xp_cb may alias res_cb.
+// With norm_partial (xpay form only) the kernel also writes one (|res|^2, 0)
+// pair per block; apply_blocks() gives the number of blocks.
 int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd, int prec, int G,
-                int isign, int tcb, const void* xp_cb, double a, hipStream_t s);
+                int isign, int tcb, const void* xp_cb, double a, hipStream_t s, double* norm_partial = nullptr);
+int apply_blocks(const int latt[4], int nd, int prec, int G);
 
 }  // namespace wd

@@ -54,10 +54,12 @@ void clear_error() {
 // launch configuration
 // ---------------------------------------------------------------------------
 // Threads per block and XCD remapping per (4D/3D, format, link reals); set by
-// dslash_set_launch (all entries) or by the autotuner (one entry).
+// dslash_set_launch or by the autotuner.  Defaults are the shapes measured
+// fastest at 32^3 x 64 on MI355X (XCD-contiguous order is worth ~25%).
+// Index: [nd == 4][F32, F64, F16, BF16][G == 12].
 static LaunchCfg g_cfg[2][4][2] = {
-    {{{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}},
-    {{{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}}};
+    {{{128, 1}, {128, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}},
+    {{{256, 1}, {256, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}}};
 
 LaunchCfg launch_cfg(int nd, int prec, int G) { return g_cfg[nd == 4][prec & 3][G == 12]; }
 void set_launch_cfg(int nd, int prec, int G, LaunchCfg c) { g_cfg[nd == 4][prec & 3][G == 12] = c; }
@@ -92,6 +94,7 @@ struct ApplyArgs {
   const v4u* xp;  // optional target-cb field of the epilogue res = xp + a * D psi
   const float* xp_norm;
   double a;
+  double* norm_partial;  // optional per-block |res|^2 (re, im=0 pairs)
   Geom g;
   int tcb;
   int remap;
@@ -118,35 +121,62 @@ __device__ __forceinline__ void hop_dir(const ApplyArgs& a, const Coord& c, uint
   }
 }
 
-template <int ND, int P, int G, int IS, bool XPAY>
+// Epilogues: EPI_PLAIN res = D psi; EPI_XPAY res = x + a D psi; EPI_XPAY_NORM also
+// writes the block's partial sum of |res|^2 to norm_partial[blockIdx.x] (the
+// solvers get <p, Mhat^dag Mhat p> = |Mhat p|^2 this way without another pass).
+enum { EPI_PLAIN = 0, EPI_XPAY = 1, EPI_XPAY_NORM = 2 };
+
+template <int ND, int P, int G, int IS, int EPI>
 __global__ void __launch_bounds__(256) k_dslash(ApplyArgs a) {
   typedef typename Fmt<P>::compute T;
   uint32_t b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   uint32_t idx = b * blockDim.x + threadIdx.x;
-  if (idx >= a.g.vh) return;
-  Coord c = site_coords<ND>(a.g, idx, a.tcb);
-  Spinor<T> acc;
+  const bool live = idx < a.g.vh;
+  if constexpr (EPI != EPI_XPAY_NORM) {
+    if (!live) return;
+  }
+  double s2 = 0.0;
+  if (live) {
+    Coord c = site_coords<ND>(a.g, idx, a.tcb);
+    Spinor<T> acc;
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc.s[s][k] = {(T)0, (T)0};
-  hop_dir<ND, P, G, IS, 0>(a, c, idx, acc);
-  hop_dir<ND, P, G, IS, 1>(a, c, idx, acc);
-  hop_dir<ND, P, G, IS, 2>(a, c, idx, acc);
-  if constexpr (ND == 4) hop_dir<ND, P, G, IS, 3>(a, c, idx, acc);
-  if constexpr (XPAY) {
-    Spinor<T> x = load_spinor<P>(a.xp, a.xp_norm, idx, a.g.vh);
-    const T s = (T)a.a;
+      for (int k = 0; k < 3; ++k) acc.s[s][k] = {(T)0, (T)0};
+    hop_dir<ND, P, G, IS, 0>(a, c, idx, acc);
+    hop_dir<ND, P, G, IS, 1>(a, c, idx, acc);
+    hop_dir<ND, P, G, IS, 2>(a, c, idx, acc);
+    if constexpr (ND == 4) hop_dir<ND, P, G, IS, 3>(a, c, idx, acc);
+    if constexpr (EPI != EPI_PLAIN) {
+      Spinor<T> x = load_spinor<P>(a.xp, a.xp_norm, idx, a.g.vh);
+      const T s = (T)a.a;
+      T n2 = (T)0;
 #pragma unroll
-    for (int sp = 0; sp < 4; ++sp)
+      for (int sp = 0; sp < 4; ++sp)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        acc.s[sp][k].re = x.s[sp][k].re + s * acc.s[sp][k].re;
-        acc.s[sp][k].im = x.s[sp][k].im + s * acc.s[sp][k].im;
-      }
-    store_spinor<P, false>(a.res, a.res_norm, idx, a.g.vh, acc);
-  } else {
-    store_spinor<P, true>(a.res, a.res_norm, idx, a.g.vh, acc);
+        for (int k = 0; k < 3; ++k) {
+          acc.s[sp][k].re = x.s[sp][k].re + s * acc.s[sp][k].re;
+          acc.s[sp][k].im = x.s[sp][k].im + s * acc.s[sp][k].im;
+          n2 += acc.s[sp][k].re * acc.s[sp][k].re + acc.s[sp][k].im * acc.s[sp][k].im;
+        }
+      s2 = (double)n2;
+      store_spinor<P, false>(a.res, a.res_norm, idx, a.g.vh, acc);
+    } else {
+      store_spinor<P, true>(a.res, a.res_norm, idx, a.g.vh, acc);
+    }
+  }
+  if constexpr (EPI == EPI_XPAY_NORM) {
+    __shared__ double lds[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += lds[w];
+      a.norm_partial[2 * blockIdx.x] = t;
+      a.norm_partial[2 * blockIdx.x + 1] = 0.0;
+    }
   }
 }
 
@@ -154,9 +184,14 @@ template <int ND, int P, int G, int IS>
 static void launch_dslash_t(const ApplyArgs& a, hipStream_t s) {
   uint32_t blocks = (a.g.vh + a.block - 1) / a.block;
   if (a.xp) {
-    if constexpr (ND == 4) hipLaunchKernelGGL((k_dslash<ND, P, G, IS, true>), dim3(blocks), dim3(a.block), 0, s, a);
+    if constexpr (ND == 4) {
+      if (a.norm_partial)
+        hipLaunchKernelGGL((k_dslash<ND, P, G, IS, EPI_XPAY_NORM>), dim3(blocks), dim3(a.block), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_dslash<ND, P, G, IS, EPI_XPAY>), dim3(blocks), dim3(a.block), 0, s, a);
+    }
   } else {
-    hipLaunchKernelGGL((k_dslash<ND, P, G, IS, false>), dim3(blocks), dim3(a.block), 0, s, a);
+    hipLaunchKernelGGL((k_dslash<ND, P, G, IS, EPI_PLAIN>), dim3(blocks), dim3(a.block), 0, s, a);
   }
 }
 
@@ -186,8 +221,14 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
   }
 }
 
+int apply_blocks(const int latt[4], int nd, int prec, int G) {
+  const uint32_t vh = (uint32_t)((uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2);
+  const int blk = launch_cfg(nd, prec, G).block;
+  return (int)((vh + blk - 1) / blk);
+}
+
 int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd, int prec, int G,
-                int isign, int tcb, const void* xp_cb, double xa, hipStream_t s) {
+                int isign, int tcb, const void* xp_cb, double xa, hipStream_t s, double* norm_partial) {
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec)) return e;
   WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12 (got %d)", G);
@@ -208,6 +249,8 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
   a.xp = (const v4u*)xp_cb;
   a.xp_norm = xp_cb ? (const float*)((const char*)xp_cb + planes) : nullptr;
   a.a = xa;
+  WD_REQUIRE(!norm_partial || xp_cb, "the |res|^2 epilogue needs the x + a*D psi form");
+  a.norm_partial = norm_partial;
   a.tcb = tcb;
   const LaunchCfg lc = launch_cfg(nd, prec, G);
   a.remap = lc.remap;
@@ -222,7 +265,7 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
 
 static int apply_cb(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd,
                     int prec, int G, int isign, int tcb, hipStream_t s) {
-  return apply_cb_ex(res_cb, psi_cb, gauge_cb, latt, nd, prec, G, isign, tcb, nullptr, 0.0, s);
+  return apply_cb_ex(res_cb, psi_cb, gauge_cb, latt, nd, prec, G, isign, tcb, nullptr, 0.0, s, nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -452,4 +452,8 @@ int blas_axpby(double a, Vec x, double b, Vec y, uint32_t vh, hipStream_t s) {
 
 int blas_scalar_op(double* scal, int op, hipStream_t s) { return finalize(nullptr, 0, scal, op, s); }
 
+int blas_finalize(double* partial, int nparts, double* scal, int op, hipStream_t s) {
+  return finalize(partial, nparts, scal, op, s);
+}
+
 }  // namespace wd

@@ -18,10 +18,12 @@ from ._lib import F16, F32, F64, CgParams, CgResult, check, latt_arg, ptr
 
 class WilsonCG:
     def __init__(self, latt, kappa: float = 0.12, tol: float = 1e-10, prec_sloppy: int = F16,
-                 gauge_reals_sloppy: int = 12, delta: float = 0.1, max_iter: int = 20000, use_graph: bool = True):
+                 gauge_reals_sloppy: int = 12, delta: float = 0.1, max_iter: int = 20000, use_graph: bool = True,
+                 gauge_reals_precise: int = 18):
         self.lib = _lib.core()
         self.latt = tuple(int(v) for v in latt)
-        self.params = CgParams(kappa, tol, delta, max_iter, prec_sloppy, gauge_reals_sloppy, 1 if use_graph else 0, 1)
+        self.params = CgParams(kappa, tol, delta, max_iter, prec_sloppy, gauge_reals_sloppy, 1 if use_graph else 0, 1,
+                               gauge_reals_precise)
         h = self.lib.dslash_cg_create(latt_arg(self.latt), C.byref(self.params))
         if not h:
             check(self.lib.dslash_last_error() or _lib.ERR_INVALID)

@@ -227,6 +227,44 @@ def mrhs_leg(reps: int = 50):
     return out
 
 
+def legacy_leg(calls: int = 6):
+    """The drop-in host-pointer API (sse_su3dslash_wilson, FP32, 32^3x64): every call
+    uploads the 604 MB gauge and the source half of psi and downloads the target
+    half of res over PCIe; and the resident-gauge extension, where only spinors move."""
+    import numpy as np
+    from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
+    from sse_wilson_dslash_amd.legacy import SSEDslash
+
+    lg = SSEDslash(32)
+    u = gen_gauge_host(LATT, 4, seed=1, dtype=np.float32)
+    psi = gen_spinor_host(LATT, 4, seed=1, dtype=np.float32)
+    res = np.zeros_like(psi)
+    lg.init_sse_su3dslash(LATT)
+    out = {}
+    try:
+        lg.sse_su3dslash_wilson(u, psi, res, 1, 0)
+        t0 = time.perf_counter()
+        for k in range(calls):
+            lg.sse_su3dslash_wilson(u, psi, res, 1 if k % 4 < 2 else -1, k % 2)
+        ms = (time.perf_counter() - t0) / calls * 1e3
+        out["host_pointers_ms_per_call"] = round(ms, 2)
+        out["host_pointers_gflops"] = round(1320 * (int(np.prod(LATT)) // 2) / (ms * 1e-3) / 1e9, 1)
+        assert lg.gauge_upload(u, 18) == 0
+        lg.wilson_resident(psi, res, 1, 0)
+        t0 = time.perf_counter()
+        for k in range(calls):
+            lg.wilson_resident(psi, res, 1 if k % 4 < 2 else -1, k % 2)
+        ms2 = (time.perf_counter() - t0) / calls * 1e3
+        out["resident_gauge_ms_per_call"] = round(ms2, 2)
+        out["resident_gauge_gflops"] = round(1320 * (int(np.prod(LATT)) // 2) / (ms2 * 1e-3) / 1e9, 1)
+    finally:
+        lg.free_sse_su3dslash()
+    log(f"[legacy] host-pointer API {out['host_pointers_ms_per_call']} ms/call "
+        f"({out['host_pointers_gflops']} GFLOP/s, PCIe-inclusive); resident gauge "
+        f"{out['resident_gauge_ms_per_call']} ms/call ({out['resident_gauge_gflops']} GFLOP/s)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +308,8 @@ def main() -> int:
         extra["mrhs"] = mrhs_leg()
     if not args.no_cg:
         extra["cg_48c96"] = cg_leg()
+    if not args.no_extra:
+        extra["legacy_api_f32_32c64"] = legacy_leg()
 
     cpu = None
     if not args.no_cpu:

@@ -126,8 +126,16 @@ __device__ __forceinline__ void hop_dir(const ApplyArgs& a, const Coord& c, uint
 // solvers get <p, Mhat^dag Mhat p> = |Mhat p|^2 this way without another pass).
 enum { EPI_PLAIN = 0, EPI_XPAY = 1, EPI_XPAY_NORM = 2 };
 
+// Minimum waves per SIMD the register allocator must leave room for: the
+// 16-bit formats have small loads that the scheduler would otherwise hoist
+// into >200 VGPRs (2 waves/SIMD); 4 waves keeps them at <= 128 without spills.
+template <int P> struct MinWaves {
+  static constexpr int value = Fmt<P>::half ? 4 : 1;
+};
+
 template <int ND, int P, int G, int IS, int EPI>
-__global__ void __launch_bounds__(256) k_dslash(ApplyArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<P>::value, 8)))
+k_dslash(ApplyArgs a) {
   typedef typename Fmt<P>::compute T;
   uint32_t b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   uint32_t idx = b * blockDim.x + threadIdx.x;
@@ -144,9 +152,14 @@ __global__ void __launch_bounds__(256) k_dslash(ApplyArgs a) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) acc.s[s][k] = {(T)0, (T)0};
     hop_dir<ND, P, G, IS, 0>(a, c, idx, acc);
+    if constexpr (Fmt<P>::half) __builtin_amdgcn_sched_barrier(0);
     hop_dir<ND, P, G, IS, 1>(a, c, idx, acc);
+    if constexpr (Fmt<P>::half) __builtin_amdgcn_sched_barrier(0);
     hop_dir<ND, P, G, IS, 2>(a, c, idx, acc);
-    if constexpr (ND == 4) hop_dir<ND, P, G, IS, 3>(a, c, idx, acc);
+    if constexpr (ND == 4) {
+      if constexpr (Fmt<P>::half) __builtin_amdgcn_sched_barrier(0);
+      hop_dir<ND, P, G, IS, 3>(a, c, idx, acc);
+    }
     if constexpr (EPI != EPI_PLAIN) {
       Spinor<T> x = load_spinor<P>(a.xp, a.xp_norm, idx, a.g.vh);
       const T s = (T)a.a;

@@ -115,6 +115,46 @@ def cpu_leg(seconds: float, threads: int):
                       f"{med * 1e3:.1f} ms/call, OpenMP over target sites"}
 
 
+def half_leg(steps: int, prec_name: str):
+    """16-bit storage Dslash (per-site norm, 12-real 16-bit links, FP32 math) at 32^3x64:
+    time, and its error against the FP64 operator on the same seeded fields."""
+    import numpy as np
+    import torch
+    from sse_wilson_dslash_amd import BF16, F16, F64
+    from sse_wilson_dslash_amd.batch import tune_apply
+    from sse_wilson_dslash_amd.device import WilsonDslash
+
+    prec = {"f16": F16, "bf16": BF16}[prec_name]
+    ref = WilsonDslash(LATT, 4, F64, 18)
+    ref.gen_gauge(seed=1)
+    psi64 = ref.gen_spinor(seed=1)
+    res64 = ref.new_spinor(zero=True)
+    ref.apply(res64, psi64, 1, 0)
+    want = ref.unpack_spinor(res64, dtype=np.float64, cbs=(1,))
+    src = ref.unpack_spinor(psi64, dtype=np.float64, cbs=(0,))
+    del ref, psi64, res64
+    torch.cuda.empty_cache()
+    op = WilsonDslash(LATT, 4, prec, 12)
+    op.gen_gauge(seed=1)
+    psi = op.pack_spinor(src)
+    res = op.new_spinor(zero=True)
+    op.apply(res, psi, 1, 0)
+    got = op.unpack_spinor(res, dtype=np.float64, cbs=(1,))
+    err = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    shape = tune_apply(op, res, psi)
+    op.time_apply(res, psi, 20)
+    ms = op.time_apply(res, psi, steps)
+    gbs = op.bytes_per_call() / (ms * 1e-3) / 1e9
+    log(f"[gpu] {prec_name} nd=4 G=12: {ms * 1e3:.1f} us/call, {op.flops_per_call() / (ms * 1e-3) / 1e9:.0f} GFLOP/s, "
+        f"{gbs:.0f} GB/s ({gbs / PEAK_HBM_GBS:.1%}), rel. error vs FP64 operator {err:.2e}")
+    out = {"gflops": round(op.flops_per_call() / (ms * 1e-3) / 1e9, 1), "us_per_call": round(ms * 1e3, 2),
+           "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4), "rel_err_vs_fp64": err,
+           "launch": shape}
+    del op, psi, res
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_leg(gib: float = 2.0, reps: int = 20):
     """Device-to-device copy bandwidth (read + write bytes / time): the practical HBM ceiling."""
     import torch
@@ -303,7 +343,10 @@ def main() -> int:
                                 "f32_3d": ("f32", 3, 18), "f64_3d": ("f64", 3, 18)}.items():
             r = gpu_leg(n, 20, p, nd, G)
             extra[key] = {"gflops": round(r["gflops"], 1), "us_per_call": round(r["ms"] * 1e3, 2),
-                          "hbm_gbs": round(r["gbs"], 1), "hbm_frac": round(r["gbs"] / PEAK_HBM_GBS, 4)}
+                          "hbm_gbs": round(r["gbs"], 1), "hbm_frac": round(r["gbs"] / PEAK_HBM_GBS, 4),
+                          "launch": r["launch"]}
+        for p in ("f16", "bf16"):
+            extra[f"{p}_4d_g12"] = half_leg(n, p)
     if not args.no_mrhs:
         extra["mrhs"] = mrhs_leg()
     if not args.no_cg:

@@ -156,8 +156,8 @@ struct Legacy {
         (e = hipMalloc(&d_u_aos, V * 4 * 18 * RS)) != hipSuccess ||
         (e = hipMalloc(&d_psi_aos, V * 24 * RS)) != hipSuccess ||
         (e = hipMalloc(&d_res_aos, V * 24 * RS)) != hipSuccess ||
-        (e = hipMalloc(&d_psi_cb, 2 * sb)) != hipSuccess ||
-        (e = hipMalloc(&d_res_cb, 2 * sb)) != hipSuccess ||
+        (e = hipMalloc(&d_psi_cb, sb)) != hipSuccess ||
+        (e = hipMalloc(&d_res_cb, sb)) != hipSuccess ||
         (e = hipMalloc(&d_bundle, 2 * gb18)) != hipSuccess ||
         (e = hipMalloc(&d_bundle_call, gb18)) != hipSuccess) {
       fprintf(stderr, "%s: device allocation failed: %s\n", name, hipGetErrorString(e));

@@ -274,8 +274,8 @@ __global__ void __launch_bounds__(TPB) k_axpby(double a, const void* x, double b
   store_spinor<P, false>((v4u*)y, vnorm<P>(y, vh), i, vh, yv);
 }
 
-__global__ void __launch_bounds__(TPB) k_finalize(const double* partial, int nparts, double* scal, int op) {
-  __shared__ double lds[TPB / 64];
+__global__ void __launch_bounds__(1024) k_finalize(const double* partial, int nparts, double* scal, int op) {
+  __shared__ double lds[1024 / 64];
   double re = 0.0, im = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
     re += partial[2 * i];
@@ -327,7 +327,7 @@ static inline int red_blocks(uint32_t vh) {
   }
 
 static int finalize(double* partial, int nparts, double* scal, int op, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(TPB), 0, s, partial, nparts, scal, op);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(nparts > 4 * TPB ? 1024 : TPB), 0, s, partial, nparts, scal, op);
   WD_LAUNCHED();
   return 0;
 }

@@ -145,8 +145,9 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, 
 
 /* ---- autotuning -------------------------------------------------------- */
 /* Time every launch shape of the single-RHS kernel (block size x XCD remap) on
- * the given fields (dslash_apply_full semantics, source cb 0; res is
- * overwritten), keep the fastest for (nd, prec, gauge_reals) and report it. */
+ * the given full-lattice fields (dslash_apply_full, alternating isign and the
+ * source checkerboard; both halves of res are overwritten), keep the fastest
+ * for (nd, prec, gauge_reals) and report it. */
 int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec,
                       int gauge_reals, void* stream, int* block_threads, int* xcd_remap, float* best_ms);
 /* Time rhs_block x block_threads (x persistent grid sizes when persistent != 0)

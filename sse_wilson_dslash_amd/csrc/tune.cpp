@@ -88,25 +88,38 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   WD_REQUIRE(prec >= F32 && prec <= BF16, "unknown precision");
   hipStream_t s = (hipStream_t)stream;
   const uint32_t vh = (uint32_t)((uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2);
-  const int reps = vh > (1u << 18) ? 20 : 200;
+  const int reps = vh > (1u << 18) ? 24 : 200;
   const LaunchCfg saved = launch_cfg(nd, prec, G);
+  // Candidates are timed in three interleaved rounds and scored by their best
+  // round, so a transient slowdown of the device does not pick a shape.
+  const int blocks[4] = {64, 128, 192, 256};
+  float tmin[4][2];
+  for (auto& row : tmin) row[0] = row[1] = 1e30f;
+  for (int round = 0; round < 3; ++round)
+    for (int bi = 0; bi < 4; ++bi)
+      for (int remap : {0, 1}) {
+        set_launch_cfg(nd, prec, G, LaunchCfg{blocks[bi], remap});
+        float t = 0;
+        int k = 0;
+        // the same alternation of isign and source checkerboard as a solver sees
+        int e = time_it(s, reps, [&]() {
+          ++k;
+          return dslash_apply_full(d_res, d_psi, d_gauge, latt, nd, prec, G, (k & 2) ? -1 : 1, k & 1, stream);
+        }, &t);
+        if (e) {
+          set_launch_cfg(nd, prec, G, saved);
+          return e;
+        }
+        if (t < tmin[bi][remap]) tmin[bi][remap] = t;
+      }
   LaunchCfg best = saved;
   float best_t = 1e30f;
-  for (int block : {64, 128, 192, 256})
-    for (int remap : {0, 1}) {
-      set_launch_cfg(nd, prec, G, LaunchCfg{block, remap});
-      float t = 0;
-      int e = time_it(s, reps, [&]() { return dslash_apply_full(d_res, d_psi, d_gauge, latt, nd, prec, G, 1, 0, stream); },
-                      &t);
-      if (e) {
-        set_launch_cfg(nd, prec, G, saved);
-        return e;
+  for (int bi = 0; bi < 4; ++bi)
+    for (int remap : {0, 1})
+      if (tmin[bi][remap] < best_t) {
+        best_t = tmin[bi][remap];
+        best = LaunchCfg{blocks[bi], remap};
       }
-      if (t < best_t) {
-        best_t = t;
-        best = LaunchCfg{block, remap};
-      }
-    }
   set_launch_cfg(nd, prec, G, best);
   if (block_threads) *block_threads = best.block;
   if (xcd_remap) *xcd_remap = best.remap;

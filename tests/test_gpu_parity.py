@@ -158,3 +158,60 @@ def test_launch_shapes_agree(oracle, block, remap):
         _lib.check(_lib.core().dslash_set_launch(4, F32, 18, b.value, r.value))
     want = oracle.dslash(u, psi, np.zeros_like(psi), latt, 1, 0)
     check(got, want, latt, 0, F32)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_xpay_epilogue(oracle, prec):
+    """dslash_apply_xpay: res = x + a D psi in one pass (and in place, x aliasing res)."""
+    from sse_wilson_dslash_amd._lib import check, ptr
+    latt = (4, 6, 4, 8)
+    dt = np.float64 if prec == F64 else np.float32
+    u = gen_gauge_host(latt, 4, seed=3, dtype=dt)
+    psi = gen_spinor_host(latt, 4, seed=3, field=1, dtype=dt)
+    x = gen_spinor_host(latt, 4, seed=3, field=2, dtype=dt)
+    a = -0.0144
+    op = WilsonDslash(latt, 4, prec, 18)
+    op.load_gauge(u)
+    vh = op.vh
+    for isign in (1, -1):
+        for tcb in (0, 1):
+            d_psi = op.pack_cb(psi, 1 - tcb)
+            d_x = op.pack_cb(x, tcb)
+            d_res = op.new_spinor(False)
+            g = ptr(op.gauge) + tcb * op.gb
+            check(op.lib.dslash_apply_xpay(ptr(d_res), ptr(d_psi), g, ptr(d_x), a, op._L, prec, 18, isign, tcb,
+                                           torch.cuda.current_stream().cuda_stream))
+            d_in = d_x.clone()
+            check(op.lib.dslash_apply_xpay(ptr(d_in), ptr(d_psi), g, ptr(d_in), a, op._L, prec, 18, isign, tcb,
+                                           torch.cuda.current_stream().cuda_stream))
+            got = op.unpack_cb(d_res, tcb, dtype=dt)
+            got_inplace = op.unpack_cb(d_in, tcb, dtype=dt)
+            d = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, 1 - tcb)
+            want = x + a * d
+            t = slice(tcb * vh, (tcb + 1) * vh)
+            assert R.rel_err(got[t], want[t]) < TOL[prec]
+            assert np.array_equal(got_inplace, got)
+
+
+@pytest.mark.parametrize("p16,tol", [(F16, 6e-4), (BF16, 4e-3)])
+def test_precision_conversion_round_trip(p16, tol):
+    """F64 -> 16-bit (per-site norm) -> F64 keeps every site to the format's rounding
+    relative to that site's largest component; F64 -> F32 -> F64 to 6e-8."""
+    from sse_wilson_dslash_amd._lib import check, ptr
+    latt = (8, 4, 4, 6)
+    psi = gen_spinor_host(latt, 4, seed=9)
+    psi[5] *= 1e-30  # a tiny site: the per-site norm keeps its relative precision
+    psi[7] = 0.0     # an all-zero site
+    op = WilsonDslash(latt, 4, F64)
+    d64 = op.pack_cb(psi, 0)
+    for pl, t in ((p16, tol), (F32, 6e-8)):
+        d_lo = op.new_spinor(False, prec=pl)
+        back = op.new_spinor(False)
+        s = torch.cuda.current_stream().cuda_stream
+        check(op.lib.dslash_convert_spinor(ptr(d_lo), pl, ptr(d64), F64, op._L, 4, s))
+        check(op.lib.dslash_convert_spinor(ptr(back), F64, ptr(d_lo), pl, op._L, 4, s))
+        got = op.unpack_cb(back, 0).reshape(-1, 24)[: op.vh]
+        ref = psi.reshape(-1, 24)[: op.vh]
+        scale = np.maximum(np.max(np.abs(ref), axis=1, keepdims=True), 1e-300)
+        assert np.max(np.abs(got - ref) / scale) < t
+        assert np.all(got[7] == 0.0)

@@ -115,16 +115,17 @@ def cpu_leg(seconds: float, threads: int):
                       f"{med * 1e3:.1f} ms/call, OpenMP over target sites"}
 
 
-def half_leg(steps: int, prec_name: str):
-    """16-bit storage Dslash (per-site norm, 12-real 16-bit links, FP32 math) at 32^3x64:
-    time, and its error against the FP64 operator on the same seeded fields."""
+def half_leg(steps: int, prec_name: str, G: int = 12):
+    """Reduced-precision Dslash at 32^3x64: 16-bit storage (per-site norm, 16-bit links,
+    FP32 math) or sloppy FP64 (FP64 spinors, FP32 links and math); time, and the error
+    against the FP64 operator on the same seeded fields."""
     import numpy as np
     import torch
-    from sse_wilson_dslash_amd import BF16, F16, F64
+    from sse_wilson_dslash_amd import BF16, F16, F64, F64S
     from sse_wilson_dslash_amd.batch import tune_apply
     from sse_wilson_dslash_amd.device import WilsonDslash
 
-    prec = {"f16": F16, "bf16": BF16}[prec_name]
+    prec = {"f16": F16, "bf16": BF16, "f64_sloppy": F64S}[prec_name]
     ref = WilsonDslash(LATT, 4, F64, 18)
     ref.gen_gauge(seed=1)
     psi64 = ref.gen_spinor(seed=1)
@@ -134,7 +135,7 @@ def half_leg(steps: int, prec_name: str):
     src = ref.unpack_spinor(psi64, dtype=np.float64, cbs=(0,))
     del ref, psi64, res64
     torch.cuda.empty_cache()
-    op = WilsonDslash(LATT, 4, prec, 12)
+    op = WilsonDslash(LATT, 4, prec, G)
     op.gen_gauge(seed=1)
     psi = op.pack_spinor(src)
     res = op.new_spinor(zero=True)
@@ -145,7 +146,7 @@ def half_leg(steps: int, prec_name: str):
     op.time_apply(res, psi, 20)
     ms = op.time_apply(res, psi, steps)
     gbs = op.bytes_per_call() / (ms * 1e-3) / 1e9
-    log(f"[gpu] {prec_name} nd=4 G=12: {ms * 1e3:.1f} us/call, {op.flops_per_call() / (ms * 1e-3) / 1e9:.0f} GFLOP/s, "
+    log(f"[gpu] {prec_name} nd=4 G={G}: {ms * 1e3:.1f} us/call, {op.flops_per_call() / (ms * 1e-3) / 1e9:.0f} GFLOP/s, "
         f"{gbs:.0f} GB/s ({gbs / PEAK_HBM_GBS:.1%}), rel. error vs FP64 operator {err:.2e}")
     out = {"gflops": round(op.flops_per_call() / (ms * 1e-3) / 1e9, 1), "us_per_call": round(ms * 1e3, 2),
            "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4), "rel_err_vs_fp64": err,
@@ -354,6 +355,7 @@ def main() -> int:
                           "launch": r["launch"]}
         for p in ("f16", "bf16"):
             extra[f"{p}_4d_g12"] = half_leg(n, p)
+        extra["f64_sloppy_4d"] = half_leg(n, "f64_sloppy", 18)
     if not args.no_mrhs:
         extra["mrhs"] = mrhs_leg()
     if not args.no_cg:

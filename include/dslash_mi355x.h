@@ -8,8 +8,8 @@
  * that already live in HBM, in the library's own layout:
  *
  *   spinor, one checkerboard:  [nvec][Vh] 16-byte vectors, the 24 reals of a
- *     site in (spin, colour, re/im) order; nvec = 6 (F32), 12 (F64), 3 (F16,
- *     BF16).  16-bit formats store x / max|x| per site and append one float
+ *     site in (spin, colour, re/im) order; nvec = 6 (F32), 12 (F64,
+ *     F64_SLOPPY), 3 (F16, BF16).  16-bit formats store x / max|x| per site and append one float
  *     norm per site after the vectors.  Size: dslash_spinor_cb_bytes().
  *     A full-lattice field is checkerboard 0 followed by checkerboard 1, each
  *     dslash_spinor_cb_bytes() long.
@@ -45,6 +45,12 @@ extern "C" {
 #define DSLASH_F64 1
 #define DSLASH_F16 2  /* binary16 spinors/links, per-site float norm, FP32 math */
 #define DSLASH_BF16 3 /* bfloat16 spinors/links, per-site float norm, FP32 math */
+/* FP64 spinors (stored exactly as DSLASH_F64) with FP32 link bundles and FP32
+ * arithmetic: the device analogue of the reference's SSEDSLASH_SLOPPY FP64
+ * build, whose half-spinor temporaries are float (include/types64.h:26-31;
+ * accepted at 1e-7 by tests/testDslashFull.cc:50-53).  Gauge bundles of this
+ * format are dslash_gauge_cb_bytes(..., DSLASH_F64_SLOPPY, G) = the F32 size. */
+#define DSLASH_F64_SLOPPY 4
 
 /* Status codes */
 #define DSLASH_OK 0

@@ -65,6 +65,15 @@ int sse_su3dslash_wilson_resident(SSEREAL* psi, SSEREAL* res, int isign, int cb)
  * hipStream_t.  Asynchronous. */
 int sse_su3dslash_wilson_device(void* d_res, const void* d_psi, int isign, int cb, void* stream);
 
+/* Sloppy FP64 (FP64 library only; DSLASH_ERR_UNSUPPORTED in the FP32 one):
+ * on = 1 keeps FP64 spinors at the interface but packs links in FP32 and runs
+ * the kernel in FP32 arithmetic (DSLASH_F64_SLOPPY), the run-time counterpart
+ * of the reference's SSEDSLASH_SLOPPY build (include/types64.h:26-31, accepted
+ * at 1e-7 by tests/testDslashFull.cc:50-53).  A library built with
+ * -DSSEDSLASH_SLOPPY starts in this mode.  Switching drops the resident gauge
+ * (upload it again). */
+int sse_su3dslash_set_sloppy(int on);
+
 /* The reference's C gauge packer (lib/packer_nopad.c:29-45, declared in no
  * reference header): reorders [mu][cb][Vh] links into the packed
  * [cb][Vh][mu] order, without transposing.  volume = V/2. */

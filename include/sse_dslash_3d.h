@@ -37,6 +37,8 @@ int sse_su3dslash_3d_gauge_upload(SSEREAL* u, int gauge_reals);
 int sse_su3dslash_3d_gauge_release(void);
 int sse_su3dslash_wilson_3d_resident(SSEREAL* psi, SSEREAL* res, int isign, int cb);
 int sse_su3dslash_wilson_3d_device(void* d_res, const void* d_psi, int isign, int cb, void* stream);
+/* Sloppy FP64 (FP64 library only), as sse_su3dslash_set_sloppy. */
+int sse_su3dslash_3d_set_sloppy(int on);
 
 #ifdef __cplusplus
 }

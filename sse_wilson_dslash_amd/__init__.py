@@ -5,6 +5,6 @@ The compute path is hand-written HIP for gfx950 behind a C ABI
 this package binds it with ctypes and uses torch only for device memory and
 streams.
 """
-from ._lib import F32, F64, F16, BF16, DslashError  # noqa: F401
+from ._lib import F32, F64, F16, BF16, F64S, DslashError  # noqa: F401
 
 __all__ = ["F32", "F64", "F16", "BF16", "DslashError"]

@@ -13,8 +13,8 @@ LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
 CORE = "libdslash_mi355x.so"
 LEGACY = {32: "libsse_dslash_f32.so", 64: "libsse_dslash_f64.so"}
 
-F32, F64, F16, BF16 = 0, 1, 2, 3
-PREC_NAMES = {F32: "f32", F64: "f64", F16: "f16", BF16: "bf16"}
+F32, F64, F16, BF16, F64S = 0, 1, 2, 3, 4
+PREC_NAMES = {F32: "f32", F64: "f64", F16: "f16", BF16: "bf16", F64S: "f64_sloppy"}
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOT_INIT, ERR_NOT_CONVERGED = range(6)
 
 _vp, _i, _sz, _u64, _f = C.c_void_p, C.c_int, C.c_size_t, C.c_uint64, C.c_float

@@ -17,7 +17,11 @@ namespace wd {
 // vectors ("planes"), so lane i of a wavefront reads site i of a plane and a
 // wavefront load is one contiguous 1 KiB segment.
 // ---------------------------------------------------------------------------
-enum Prec : int { F32 = 0, F64 = 1, F16 = 2, BF16 = 3 };
+// F64S ("sloppy FP64") keeps FP64 spinors but FP32 links and FP32 arithmetic:
+// the on-device analogue of the reference's SSEDSLASH_SLOPPY build, whose FP64
+// Dslash keeps its half-spinor temporaries in float (include/types64.h:26-31).
+enum Prec : int { F32 = 0, F64 = 1, F16 = 2, BF16 = 3, F64S = 4 };
+constexpr int NUM_PREC = 5;
 
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
@@ -46,10 +50,24 @@ template <> struct Fmt<BF16> {
   static constexpr int bytes = 2;
   static constexpr bool half = true;
 };
+template <> struct Fmt<F64S> {
+  typedef double real;
+  typedef float compute;
+  static constexpr int bytes = 8;
+  static constexpr bool half = false;
+};
 
+// Bytes of one stored spinor real.
 __host__ __device__ constexpr int prec_bytes(int p) {
-  return p == F64 ? 8 : (p == F32 ? 4 : 2);
+  return (p == F64 || p == F64S) ? 8 : (p == F32 ? 4 : 2);
 }
+// Format of the link bundle that goes with a spinor format, and of spinor
+// storage (F64S spinors are stored exactly as F64 ones).
+__host__ __device__ constexpr int link_prec(int p) { return p == F64S ? F32 : p; }
+__host__ __device__ constexpr int spinor_store_prec(int p) { return p == F64S ? F64 : p; }
+template <int P> struct LinkP {
+  static constexpr int value = link_prec(P);
+};
 __host__ __device__ constexpr bool prec_is_half(int p) { return p == F16 || p == BF16; }
 
 // 16-bit float conversions.  FP16 uses the compiler's _Float16 (IEEE binary16,
@@ -76,12 +94,14 @@ template <> __host__ __device__ inline float to_compute<F32>(float v) { return v
 template <> __host__ __device__ inline double to_compute<F64>(double v) { return v; }
 template <> __host__ __device__ inline float to_compute<F16>(uint16_t v) { return h16_to_f(v); }
 template <> __host__ __device__ inline float to_compute<BF16>(uint16_t v) { return bf16_to_f(v); }
+template <> __host__ __device__ inline float to_compute<F64S>(double v) { return (float)v; }
 
 template <int P> __host__ __device__ inline typename Fmt<P>::real from_compute(typename Fmt<P>::compute v);
 template <> __host__ __device__ inline float from_compute<F32>(float v) { return v; }
 template <> __host__ __device__ inline double from_compute<F64>(double v) { return v; }
 template <> __host__ __device__ inline uint16_t from_compute<F16>(float v) { return f_to_h16(v); }
 template <> __host__ __device__ inline uint16_t from_compute<BF16>(float v) { return f_to_bf16(v); }
+template <> __host__ __device__ inline double from_compute<F64S>(float v) { return (double)v; }
 
 // ---------------------------------------------------------------------------
 // Complex arithmetic on plain structs (no std::complex in device code).
@@ -313,9 +333,9 @@ template <typename T> __host__ __device__ inline void su3_reconstruct_row2(C<T> 
 // link keeps G = 18 reals (full 3x3) or G = 12 (rows 0 and 1, row 2 rebuilt).
 // ---------------------------------------------------------------------------
 template <int P, int G> struct PairVec {
-  static constexpr int value = (2 * G * Fmt<P>::bytes + 15) / 16;
+  static constexpr int value = (2 * G * Fmt<LinkP<P>::value>::bytes + 15) / 16;
 };
-__host__ __device__ constexpr int pair_vec(int prec, int G) { return (2 * G * prec_bytes(prec) + 15) / 16; }
+__host__ __device__ constexpr int pair_vec(int prec, int G) { return (2 * G * prec_bytes(link_prec(prec)) + 15) / 16; }
 __host__ __device__ constexpr int spinor_vec(int prec) { return 24 * prec_bytes(prec) / 16; }
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -396,9 +416,11 @@ template <int P, bool NT> __device__ inline void store_spinor(v4u* __restrict__ 
   }
 }
 
-// Load the two links of direction mu for one target site.
+// Load the two links of direction mu for one target site (links are stored in
+// the format link_prec(P) and converted to P's arithmetic type).
 template <int P, int G> __device__ inline void load_pair(const v4u* __restrict__ pair_base, uint32_t idx, uint32_t vh, C<typename Fmt<P>::compute> Wf[3][3], C<typename Fmt<P>::compute> Wb[3][3]) {
-  typedef typename Fmt<P>::real R;
+  constexpr int PL = LinkP<P>::value;
+  typedef typename Fmt<PL>::real R;
   constexpr int NV = PairVec<P, G>::value;
   v4u buf[NV];
 #pragma unroll
@@ -409,10 +431,10 @@ template <int P, int G> __device__ inline void load_pair(const v4u* __restrict__
   for (int i = 0; i < ROWS; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      Wf[i][j].re = to_compute<P>(r[2 * (3 * i + j)]);
-      Wf[i][j].im = to_compute<P>(r[2 * (3 * i + j) + 1]);
-      Wb[i][j].re = to_compute<P>(r[G + 2 * (3 * i + j)]);
-      Wb[i][j].im = to_compute<P>(r[G + 2 * (3 * i + j) + 1]);
+      Wf[i][j].re = to_compute<PL>(r[2 * (3 * i + j)]);
+      Wf[i][j].im = to_compute<PL>(r[2 * (3 * i + j) + 1]);
+      Wb[i][j].re = to_compute<PL>(r[G + 2 * (3 * i + j)]);
+      Wb[i][j].im = to_compute<PL>(r[G + 2 * (3 * i + j) + 1]);
     }
   if (G == 12) {
     su3_reconstruct_row2(Wf);

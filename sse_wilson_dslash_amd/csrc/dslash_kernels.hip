@@ -56,13 +56,14 @@ void clear_error() {
 // Threads per block and XCD remapping per (4D/3D, format, link reals); set by
 // dslash_set_launch or by the autotuner.  Defaults are the shapes measured
 // fastest at 32^3 x 64 on MI355X (XCD-contiguous order is worth ~25%).
-// Index: [nd == 4][F32, F64, F16, BF16][G == 12].
-static LaunchCfg g_cfg[2][4][2] = {
-    {{{128, 1}, {128, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}},
-    {{{256, 1}, {256, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}}};
+// Index: [nd == 4][F32, F64, F16, BF16, F64S][G == 12].
+static LaunchCfg g_cfg[2][NUM_PREC][2] = {
+    {{{128, 1}, {128, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{128, 1}, {128, 1}}},
+    {{{256, 1}, {256, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}}};
 
-LaunchCfg launch_cfg(int nd, int prec, int G) { return g_cfg[nd == 4][prec & 3][G == 12]; }
-void set_launch_cfg(int nd, int prec, int G, LaunchCfg c) { g_cfg[nd == 4][prec & 3][G == 12] = c; }
+static int prec_slot(int prec) { return prec >= 0 && prec < NUM_PREC ? prec : 0; }
+LaunchCfg launch_cfg(int nd, int prec, int G) { return g_cfg[nd == 4][prec_slot(prec)][G == 12]; }
+void set_launch_cfg(int nd, int prec, int G, LaunchCfg c) { g_cfg[nd == 4][prec_slot(prec)][G == 12] = c; }
 
 static int check_lattice(const int latt[4], int nd) {
   WD_REQUIRE(latt != nullptr, "latt is NULL");
@@ -78,7 +79,7 @@ static int check_lattice(const int latt[4], int nd) {
 }
 
 static int check_prec(int prec) {
-  WD_REQUIRE(prec == F32 || prec == F64 || prec == F16 || prec == BF16, "unknown precision %d", prec);
+  WD_REQUIRE(prec == F32 || prec == F64 || prec == F16 || prec == BF16 || prec == F64S, "unknown precision %d", prec);
   return 0;
 }
 
@@ -230,6 +231,7 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
     case F32: launch_dslash_p<ND, F32>(a, G, isign, s); break;
     case F64: launch_dslash_p<ND, F64>(a, G, isign, s); break;
     case F16: launch_dslash_p<ND, F16>(a, G, isign, s); break;
+    case F64S: launch_dslash_p<ND, F64S>(a, G, isign, s); break;
     default: launch_dslash_p<ND, BF16>(a, G, isign, s); break;
   }
 }
@@ -479,7 +481,7 @@ int dslash_set_launch(int nd, int prec, int gauge_reals, int block_threads, int 
   LaunchCfg c{block_threads, xcd_remap ? 1 : 0};
   if (nd == 0) {
     for (int n : {3, 4})
-      for (int p = 0; p < 4; ++p)
+      for (int p = 0; p < NUM_PREC; ++p)
         for (int G : {18, 12}) set_launch_cfg(n, p, G, c);
     return 0;
   }
@@ -535,6 +537,7 @@ int dslash_pack_gauge(void* d_bundle, int target_cb, const void* d_u_packed, int
   clear_error();
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec_out)) return e;
+  prec_out = link_prec(prec_out);
   WD_REQUIRE(prec_in == F32 || prec_in == F64, "packed gauge must be F32 or F64");
   WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
   WD_REQUIRE(target_cb == 0 || target_cb == 1, "target_cb must be 0 or 1");
@@ -572,6 +575,7 @@ int dslash_pack_spinor(void* d_cb, int cb, const void* d_aos, int prec_in, const
   clear_error();
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec_out)) return e;
+  prec_out = spinor_store_prec(prec_out);
   WD_REQUIRE(prec_in == F32 || prec_in == F64, "legacy spinor must be F32 or F64");
   WD_REQUIRE(cb == 0 || cb == 1, "cb must be 0 or 1");
   WD_REQUIRE(d_cb && d_aos, "NULL pointer");
@@ -603,6 +607,7 @@ int dslash_unpack_spinor(void* d_aos, int cb, const void* d_cb, int prec_cb, con
   clear_error();
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec_cb)) return e;
+  prec_cb = spinor_store_prec(prec_cb);
   WD_REQUIRE(prec_out == F32 || prec_out == F64, "legacy spinor must be F32 or F64");
   WD_REQUIRE(cb == 0 || cb == 1, "cb must be 0 or 1");
   WD_REQUIRE(d_cb && d_aos, "NULL pointer");
@@ -635,6 +640,8 @@ int dslash_convert_spinor(void* d_out, int prec_out, const void* d_in, int prec_
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec_out)) return e;
   if (int e = check_prec(prec_in)) return e;
+  prec_out = spinor_store_prec(prec_out);
+  prec_in = spinor_store_prec(prec_in);
   WD_REQUIRE(d_out && d_in && d_out != d_in, "bad pointers");
   Geom g = make_geom(latt, nd);
   const v4u* in = (const v4u*)d_in;

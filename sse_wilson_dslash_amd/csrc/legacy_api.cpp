@@ -27,6 +27,13 @@
 #else
 #define LEG_PREC DSLASH_F64
 #endif
+// A build with -DSSEDSLASH_SLOPPY (the reference's --enable-sloppy) starts the
+// FP64 library in sloppy mode; sse_su3dslash_set_sloppy switches at run time.
+#if SSE_PRECISION == 64 && defined(SSEDSLASH_SLOPPY)
+#define LEG_DEV_PREC DSLASH_F64_SLOPPY
+#else
+#define LEG_DEV_PREC LEG_PREC
+#endif
 
 namespace {
 
@@ -51,6 +58,7 @@ struct Legacy {
   void* d_bundle_call = nullptr;  // per-call bundle of one target cb
   size_t sb = 0, gb18 = 0;
   int resident_G = 0;        // 0: nothing resident; 18/12: both bundles hold the uploaded gauge
+  int prec = LEG_DEV_PREC;   // device format of bundles and kernel (LEG_PREC or F64_SLOPPY)
   hipStream_t stream = nullptr;
   std::vector<SSEREAL> h_stage;
 
@@ -206,7 +214,7 @@ struct Legacy {
     hipError_t e = hipMemcpyAsync(d_u_aos, u, V * 4 * 18 * RS, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, "gauge upload");
     for (int t = 0; t < 2; ++t)
-      if (slot[t] && dslash_pack_gauge(slot[t], t, d_u_aos, LEG_PREC, latt, nd, LEG_PREC, G, d_site_table, stream))
+      if (slot[t] && dslash_pack_gauge(slot[t], t, d_u_aos, LEG_PREC, latt, nd, prec, G, d_site_table, stream))
         return fail(dslash_last_error(), "gauge pack");
     return 0;
   }
@@ -221,19 +229,19 @@ struct Legacy {
     else
       e = hipMemcpyAsync(d_psi_aos, psi, V * 24 * RS, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, "spinor upload");
-    if (dslash_pack_spinor(d_psi_cb, cb, d_psi_aos, LEG_PREC, latt, nd, LEG_PREC, d_site_table, stream))
+    if (dslash_pack_spinor(d_psi_cb, cb, d_psi_aos, LEG_PREC, latt, nd, prec, d_site_table, stream))
       return fail(dslash_last_error(), "spinor pack");
-    if (dslash_apply(d_res_cb, d_psi_cb, bundle_t, latt, nd, LEG_PREC, G, isign, tcb, stream))
+    if (dslash_apply(d_res_cb, d_psi_cb, bundle_t, latt, nd, prec, G, isign, tcb, stream))
       return fail(dslash_last_error(), "dslash");
     if (canonical) {
-      if (dslash_unpack_spinor(d_res_aos, tcb, d_res_cb, LEG_PREC, latt, nd, LEG_PREC, nullptr, 0, stream))
+      if (dslash_unpack_spinor(d_res_aos, tcb, d_res_cb, prec, latt, nd, LEG_PREC, nullptr, 0, stream))
         return fail(dslash_last_error(), "spinor unpack");
       e = hipMemcpyAsync(res + (size_t)tcb * Vh * 24, (char*)d_res_aos + (size_t)tcb * Vh * 24 * RS, Vh * 24 * RS,
                          hipMemcpyDeviceToHost, stream);
       if (e != hipSuccess) return hip_fail(e, "result download");
       if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "synchronize");
     } else {
-      if (dslash_unpack_spinor(d_res_aos, tcb, d_res_cb, LEG_PREC, latt, nd, LEG_PREC, nullptr, 1, stream))
+      if (dslash_unpack_spinor(d_res_aos, tcb, d_res_cb, prec, latt, nd, LEG_PREC, nullptr, 1, stream))
         return fail(dslash_last_error(), "spinor unpack");
       e = hipMemcpyAsync(h_stage.data(), d_res_aos, Vh * 24 * RS, hipMemcpyDeviceToHost, stream);
       if (e != hipSuccess) return hip_fail(e, "result download");
@@ -260,7 +268,7 @@ struct Legacy {
   int gauge_upload(const SSEREAL* u, int G) {
     if (int r = check_ready()) return r;
     if (G != 18 && G != 12) return fail(DSLASH_ERR_INVALID, "gauge_reals must be 18 or 12");
-    size_t gb = dslash_gauge_cb_bytes(latt, nd, LEG_PREC, G);
+    size_t gb = dslash_gauge_cb_bytes(latt, nd, prec, G);
     void* slots[2] = {d_bundle, (char*)d_bundle + gb};
     if (int r = upload_gauge(u, G, slots)) return r;
     hipError_t e = hipStreamSynchronize(stream);
@@ -275,19 +283,30 @@ struct Legacy {
     return 0;
   }
 
+  // Sloppy FP64: FP64 spinors, FP32 links and arithmetic.  The bundles change
+  // format, so a resident gauge is dropped and must be uploaded again.
+  int set_sloppy(int on) {
+    if (on && LEG_PREC != DSLASH_F64) return fail(DSLASH_ERR_UNSUPPORTED, "sloppy mode exists for the FP64 library only");
+    int p = on ? DSLASH_F64_SLOPPY : LEG_PREC;
+    if (p != prec) resident_G = 0;
+    prec = p;
+    last_error = 0;
+    return 0;
+  }
+
   int wilson_resident(SSEREAL* psi, SSEREAL* res, int isign, int cb) {
     if (int r = check_ready()) return r;
     if (isign != 1 && isign != -1) return fail(DSLASH_ERR_INVALID, "isign must be +1 or -1");
     if (cb != 0 && cb != 1) return fail(DSLASH_ERR_INVALID, "cb must be 0 or 1");
     if (!resident_G) return fail(DSLASH_ERR_NOT_INIT, "no resident gauge: call *_gauge_upload first");
-    size_t gb = dslash_gauge_cb_bytes(latt, nd, LEG_PREC, resident_G);
+    size_t gb = dslash_gauge_cb_bytes(latt, nd, prec, resident_G);
     return apply_host(psi, res, isign, cb, (char*)d_bundle + (size_t)(1 - cb) * gb, resident_G);
   }
 
   int wilson_device(void* d_res, const void* d_psi, int isign, int cb, void* s) {
     if (int r = check_ready()) return r;
     if (!resident_G) return fail(DSLASH_ERR_NOT_INIT, "no resident gauge: call *_gauge_upload first");
-    if (dslash_apply_full(d_res, d_psi, d_bundle, latt, nd, LEG_PREC, resident_G, isign, cb, s))
+    if (dslash_apply_full(d_res, d_psi, d_bundle, latt, nd, prec, resident_G, isign, cb, s))
       return fail(dslash_last_error(), "dslash");
     last_error = 0;
     return 0;
@@ -320,6 +339,7 @@ int sse_su3dslash_wilson_resident(SSEREAL* psi, SSEREAL* res, int isign, int cb)
 int sse_su3dslash_wilson_device(void* d_res, const void* d_psi, int isign, int cb, void* stream) {
   return g4.wilson_device(d_res, d_psi, isign, cb, stream);
 }
+int sse_su3dslash_set_sloppy(int on) { return g4.set_sloppy(on); }
 
 void init_sse_su3dslash_3d(const int* latt_size, void (*getSiteCoords)(int coord[], int node, int linearsite),
                            int (*getLinearSiteIndex)(const int coord[]), int (*nodeNumber)(const int coord[])) {
@@ -339,6 +359,7 @@ int sse_su3dslash_wilson_3d_resident(SSEREAL* psi, SSEREAL* res, int isign, int 
 int sse_su3dslash_wilson_3d_device(void* d_res, const void* d_psi, int isign, int cb, void* stream) {
   return g3.wilson_device(d_res, d_psi, isign, cb, stream);
 }
+int sse_su3dslash_3d_set_sloppy(int on) { return g3.set_sloppy(on); }
 
 // lib/packer_nopad.c:29-45: u is [mu][cb][volume], u_tmp is [cb][volume][mu].
 void pack_gauge_field(int volume, sse_u_mat_array* u, sse_u_mat_array* u_tmp) {

@@ -85,7 +85,7 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   clear_error();
   if (int e = dslash_check_lattice(latt, nd)) return e;
   WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
-  WD_REQUIRE(prec >= F32 && prec <= BF16, "unknown precision");
+  WD_REQUIRE(prec >= F32 && prec < NUM_PREC, "unknown precision");
   hipStream_t s = (hipStream_t)stream;
   const uint32_t vh = (uint32_t)((uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2);
   const int reps = vh > (1u << 18) ? 24 : 200;
@@ -166,7 +166,7 @@ int dslash_tune_save(const char* path) {
   WD_REQUIRE(f, "cannot write %s", path);
   fprintf(f, "# dslash_mi355x launch shapes\n");
   for (int nd : {3, 4})
-    for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < NUM_PREC; ++p)
       for (int G : {18, 12}) {
         LaunchCfg c = launch_cfg(nd, p, G);
         fprintf(f, "apply %d %d %d %d %d\n", nd, p, G, c.block, c.remap);
@@ -192,7 +192,7 @@ int dslash_tune_load(const char* path) {
     int a[6];
     if (line[0] == '#') continue;
     if (sscanf(line, "apply %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4]) == 5) {
-      if ((a[0] == 3 || a[0] == 4) && a[1] >= 0 && a[1] < 4 && (a[2] == 18 || a[2] == 12) && a[3] >= 64 &&
+      if ((a[0] == 3 || a[0] == 4) && a[1] >= 0 && a[1] < NUM_PREC && (a[2] == 18 || a[2] == 12) && a[3] >= 64 &&
           a[3] <= 256 && a[3] % 64 == 0)
         set_launch_cfg(a[0], a[1], a[2], LaunchCfg{a[3], a[4] ? 1 : 0});
       else

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import F32, F64, F16, BF16, check, latt_arg, ptr
+from ._lib import F32, F64, F16, BF16, F64S, check, latt_arg, ptr
 
 _NP = {F32: np.float32, F64: np.float64}
 _TORCH = {F32: torch.float32, F64: torch.float64}
@@ -37,7 +37,8 @@ class WilsonDslash:
 
     nd = 4: sum over mu = 0..3 in the cb2 order; nd = 3: mu = 0..2 in the cb3d
     order.  prec: F32, F64, F16 or BF16 (16-bit formats store spinors with a
-    per-site norm and links as raw values, arithmetic in FP32).  gauge_reals:
+    per-site norm and links as raw values, arithmetic in FP32), or F64S (FP64
+    spinors, FP32 links and arithmetic: the sloppy FP64 operator).  gauge_reals:
     18 (full links) or 12 (two rows, SU(3) links only).
     """
 
@@ -98,7 +99,7 @@ class WilsonDslash:
 
     def gen_gauge(self, seed: int = 1) -> None:
         """Generate the seeded SU(3) gauge directly in HBM and pack it."""
-        pin = F64 if self.prec == F64 else F32
+        pin = F64 if self.prec in (F64, F64S) else F32
         d_u = torch.empty(self.V * 72, dtype=_TORCH[pin], device=self.device)
         check(self.lib.dslash_gen_gauge_device(ptr(d_u), pin, self._L, self.nd, seed, current_stream()))
         self.load_gauge(d_u)
@@ -164,7 +165,7 @@ class WilsonDslash:
 
     def gen_spinor(self, seed: int = 1, field: int = 0, prec: int | None = None) -> torch.Tensor:
         """Seeded Gaussian spinor generated in HBM, returned in device layout."""
-        pin = F64 if (self.prec if prec is None else prec) == F64 else F32
+        pin = F64 if (self.prec if prec is None else prec) in (F64, F64S) else F32
         d = torch.empty(self.V * 24, dtype=_TORCH[pin], device=self.device)
         check(self.lib.dslash_gen_spinor_device(ptr(d), pin, self._L, self.nd, seed, field, current_stream()))
         return self.pack_spinor(d, prec=prec)
@@ -201,9 +202,10 @@ class WilsonDslash:
 
     def bytes_per_call(self) -> int:
         """Compulsory HBM bytes: read one source spinor and 2*nd links, write one spinor, per target site."""
-        s = {F32: 4, F64: 8, F16: 2, BF16: 2}[self.prec]
+        s = {F32: 4, F64: 8, F16: 2, BF16: 2, F64S: 8}[self.prec]
+        sl = 4 if self.prec == F64S else s  # F64S: FP32 links
         norm = 8 if self.prec in (F16, BF16) else 0
-        return self.vh * (48 * s + 2 * self.nd * self.G * s + norm)
+        return self.vh * (48 * s + 2 * self.nd * self.G * sl + norm)
 
 
 def gen_gauge_host(latt, nd: int = 4, seed: int = 1, dtype=np.float64) -> np.ndarray:

@@ -85,6 +85,9 @@ class SSEDslash:
                      "sse_su3dslash_gauge_release", "sse_su3dslash_3d_gauge_release"):
             getattr(L, name).restype = _i
             getattr(L, name).argtypes = []
+        for name in ("sse_su3dslash_set_sloppy", "sse_su3dslash_3d_set_sloppy"):
+            getattr(L, name).restype = _i
+            getattr(L, name).argtypes = [_i]
         for name in ("sse_su3dslash_gauge_upload", "sse_su3dslash_3d_gauge_upload"):
             getattr(L, name).restype = _i
             getattr(L, name).argtypes = [_vp, _i]
@@ -155,6 +158,13 @@ class SSEDslash:
     def wilson_device(self, d_res, d_psi, isign, cb, stream=None) -> int:
         return self.lib.sse_su3dslash_wilson_device(_lib.ptr(d_res), _lib.ptr(d_psi), int(isign), int(cb),
                                                     stream)
+
+    def set_sloppy(self, on: bool = True) -> int:
+        """FP64 library: FP32 links and arithmetic behind the FP64 interface."""
+        return self.lib.sse_su3dslash_set_sloppy(int(bool(on)))
+
+    def set_sloppy_3d(self, on: bool = True) -> int:
+        return self.lib.sse_su3dslash_3d_set_sloppy(int(bool(on)))
 
     # ---- 3D ------------------------------------------------------------
     def init_sse_su3dslash_3d(self, latt_size, getSiteCoords=None, getLinearSiteIndex=None, nodeNumber=None):

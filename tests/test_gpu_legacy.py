@@ -64,6 +64,52 @@ def test_legacy_canonical(oracle, libs, prec, nd):
 
 
 @pytest.mark.parametrize("nd", [4, 3])
+def test_legacy_sloppy_fp64(oracle, libs, nd):
+    """The FP64 library in sloppy mode (FP32 links and arithmetic behind FP64
+    arrays) meets the reference's SSEDSLASH_SLOPPY acceptance, diff_norm < 1e-7
+    (tests/testDslashFull.cc:50-53), on the host, resident and 12-real paths;
+    switching back restores FP64 parity."""
+    lg = libs[64]
+    latt = (8, 8, 8, 8) if nd == 4 else (8, 4, 6, 5)
+    print(f"[legacy sloppy] nd={nd}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=3)
+    psi = gen_spinor_host(latt, nd, seed=3)
+    init = lg.init_sse_su3dslash if nd == 4 else lg.init_sse_su3dslash_3d
+    free = lg.free_sse_su3dslash if nd == 4 else lg.free_sse_su3dslash_3d
+    apply = lg.sse_su3dslash_wilson if nd == 4 else lg.sse_su3dslash_wilson_3d
+    sloppy = lg.set_sloppy if nd == 4 else lg.set_sloppy_3d
+    upload = lg.gauge_upload if nd == 4 else lg.gauge_upload_3d
+    resident = lg.wilson_resident if nd == 4 else lg.wilson_3d_resident
+    init(latt)
+    try:
+        assert sloppy(True) == 0
+        for path in ("host", "resident18", "resident12"):
+            if path != "host":
+                assert upload(u, 18 if path == "resident18" else 12) == 0
+            for isign in (1, -1):
+                for cb in (0, 1):
+                    res = np.zeros_like(psi)
+                    if path == "host":
+                        apply(u, psi, res, isign, cb)
+                    else:
+                        assert resident(psi, res, isign, cb) == 0
+                    want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
+                    e = R.rel_err(res, want)
+                    assert diff_norm(res, want, latt) < 1e-7 and 1e-12 < e < 1e-6, (path, isign, cb, e)
+        assert sloppy(False) == 0
+        # the resident gauge was packed for the sloppy format: it is dropped
+        assert resident(psi, np.zeros_like(psi), 1, 0) != 0
+        assert upload(u, 18) == 0
+        res = np.zeros_like(psi)
+        assert resident(psi, res, 1, 0) == 0
+        want = oracle.dslash(u, psi, np.zeros_like(psi), latt, 1, 0, nd=nd)
+        assert R.rel_err(res, want) < 1e-13
+    finally:
+        sloppy(False)
+        free()
+
+
+@pytest.mark.parametrize("nd", [4, 3])
 def test_legacy_caller_order_callbacks(oracle, libs, nd):
     """A non-canonical caller order supplied through the callbacks."""
     lg = libs[64]

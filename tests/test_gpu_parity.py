@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-from sse_wilson_dslash_amd import F16, F32, F64, BF16
+from sse_wilson_dslash_amd import F16, F32, F64, BF16, F64S
 from sse_wilson_dslash_amd.device import WilsonDslash, gen_gauge_host, gen_spinor_host
 from sse_wilson_dslash_amd.layout import volume
 from tests import refimpl as R
@@ -32,7 +32,7 @@ def run_gpu(u, psi, latt, nd, prec, G, isign, cb, sentinel=None):
     d_res = op.pack_spinor(sentinel if sentinel is not None else np.zeros_like(psi))
     op.apply(d_res, d_psi, isign, cb)
     torch.cuda.synchronize()
-    dt = np.float64 if prec == F64 else np.float32
+    dt = np.float64 if prec in (F64, F64S) else np.float32
     return op.unpack_spinor(d_res, dtype=dt)
 
 
@@ -105,6 +105,38 @@ def test_full_size_32c64(oracle, nd, prec, G):
             oracle.dslash(u, psi, want, latt, isign, cb, nd=nd)
             e = check(got, want, latt, cb, prec, f"isign={isign} cb={cb}")
             print(f"  isign={isign:+d} cb={cb} rel_err={e:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("nd,latt", [(4, (4, 4, 4, 4)), (4, (6, 2, 2, 10)), (3, (4, 2, 6, 3)),
+                                     (4, (16, 16, 16, 32)), (3, (16, 16, 16, 8))])
+@pytest.mark.parametrize("G", [18, 12])
+def test_sloppy_fp64(oracle, nd, latt, G):
+    """F64S = FP64 spinors at the interface, FP32 links and FP32 arithmetic: the
+    reference's SSEDSLASH_SLOPPY FP64 build (include/types64.h:26-31), which its
+    test accepts at diff_norm < 1e-7 (tests/testDslashFull.cc:50-53, 103-109).
+    Bound here: relative L2 and max error against the FP64 oracle <= 1e-6, and
+    equal to the F32 kernel on the same (rounded) inputs to FP32 rounding."""
+    print(f"[sloppy fp64] nd={nd} latt={latt} G={G}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=2)
+    psi = gen_spinor_host(latt, nd, seed=2, field=1)
+    vh = volume(latt) // 2
+    for isign in (1, -1):
+        for cb in (0, 1):
+            sentinel = np.full_like(psi, 5.5)
+            got = run_gpu(u, psi, latt, nd, F64S, G, isign, cb, sentinel)
+            assert got.dtype == np.float64
+            want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
+            t = slice((1 - cb) * vh, (2 - cb) * vh)
+            g, w = got.reshape(-1, 24)[t], want.reshape(-1, 24)[t]
+            e2, einf = R.rel_err(g, w), R.max_rel_err(g, w)
+            dn = np.sqrt(np.sum((g - w) ** 2)) / (24 * vh)
+            print(f"  isign={isign:+d} cb={cb} rel L2 {e2:.2e} rel max {einf:.2e} diff_norm {dn:.2e}", flush=True)
+            assert e2 < 1e-6 and einf < 1e-6 and dn < 1e-7
+            assert e2 > 1e-12, "sloppy result is suspiciously exact: FP32 path not taken"
+            assert np.all(got.reshape(-1, 24)[cb * vh:(cb + 1) * vh] == 5.5), "source cb of res modified"
+            f32 = run_gpu(u.astype(np.float32), psi.astype(np.float32), latt, nd, F32, G, isign, cb)
+            d = np.max(np.abs(g - f32.reshape(-1, 24)[t].astype(np.float64))) / np.max(np.abs(w))
+            assert d < 1e-6, f"F64S vs F32 kernel: {d:.2e}"
 
 
 @pytest.mark.parametrize("prec,tol", [(F16, 2e-3), (BF16, 1.6e-2)])

@@ -127,6 +127,11 @@ def test_sizes_and_validation():
     assert lib.dslash_gauge_cb_bytes(L, 4, _lib.F32, 12) == vh * 8 * 12 * 4
     assert lib.dslash_gauge_cb_bytes(L, 3, _lib.F64, 18) == vh * 6 * 18 * 8
     assert lib.dslash_gauge_cb_bytes(L, 4, _lib.F16, 12) == vh * 8 * 12 * 2
+    # sloppy FP64: FP64 spinor storage, FP32 link bundles
+    assert lib.dslash_spinor_cb_bytes(L, 4, _lib.F64S) == vh * 192
+    assert lib.dslash_gauge_cb_bytes(L, 4, _lib.F64S, 18) == vh * 8 * 18 * 4
+    assert lib.dslash_gauge_cb_bytes(L, 3, _lib.F64S, 12) == vh * 6 * 12 * 4
+    assert lib.dslash_spinor_cb_bytes(L, 4, 5) == 0
     assert lib.dslash_check_lattice(_lib.latt_arg((4, 4, 4, 3)), 4) == _lib.ERR_INVALID
     assert b"even" in lib.dslash_last_error_message()
     assert lib.dslash_check_lattice(_lib.latt_arg((4, 4, 4, 3)), 3) == 0
@@ -169,6 +174,18 @@ def test_tune_file_round_trip(tmp_path):
     bad.write_text("apply 4 0 18 100 1\n")
     assert lib.dslash_tune_load(str(bad).encode()) == _lib.ERR_INVALID
     assert lib.dslash_tune_load(b"/nonexistent/tune.txt") == _lib.ERR_INVALID
+
+
+def test_sloppy_switch_host_logic():
+    """sse_su3dslash_set_sloppy exists in both libraries; only the FP64 one
+    accepts it (no device work is done by the switch itself)."""
+    from sse_wilson_dslash_amd.legacy import SSEDslash
+    l32, l64 = SSEDslash(32), SSEDslash(64)
+    assert l32.set_sloppy(True) == _lib.ERR_UNSUPPORTED and l32.last_error() == _lib.ERR_UNSUPPORTED
+    assert l32.set_sloppy(False) == 0 and l32.last_error() == 0
+    assert l32.set_sloppy_3d(True) == _lib.ERR_UNSUPPORTED
+    assert l64.set_sloppy(True) == 0 and l64.set_sloppy(False) == 0
+    assert l64.set_sloppy_3d(True) == 0 and l64.set_sloppy_3d(False) == 0
 
 
 @pytest.mark.parametrize("prec", [32, 64])

@@ -189,6 +189,19 @@ def test_sloppy_switch_host_logic():
 
 
 @pytest.mark.parametrize("prec", [32, 64])
+def test_qdp_packer_headers(tmp_path, prec):
+    """include/sse_dslash_qdp_packer*.h compile against a minimal QDP++-shaped
+    stub (tests/qdp_stub) and write the transposed [site][mu] wire format."""
+    exe = tmp_path / f"qpc{prec}"
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", f"-DSSE_PRECISION={prec}",
+           "-I", os.path.join(ROOT, "tests", "qdp_stub"), "-I", os.path.join(ROOT, "include"),
+           "-o", str(exe), os.path.join(ROOT, "tests", "qdp_packer_check.cpp")]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and out.stdout.startswith("OK"), out.stdout + out.stderr
+
+
+@pytest.mark.parametrize("prec", [32, 64])
 def test_c_gauge_packer(prec):
     """pack_gauge_field reorders [mu][cb][Vh] -> [cb][Vh][mu] without transposing
     (lib/packer_nopad.c:29-45); unpack_gauge_field is its inverse."""

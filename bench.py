@@ -156,6 +156,48 @@ def half_leg(steps: int, prec_name: str, G: int = 12):
     return out
 
 
+def halo_leg(reps: int = 200):
+    """Domain-decomposition split (face pack, interior, exterior) with T partitioned, at the
+    local shape of an 8-way T split of 32^3x64 (32^3x8) and at 32^3x64; one GPU, so the
+    exchange is the identity.  Event timing on the launch stream; the faces' cost is
+    split - interior."""
+    import torch
+    from sse_wilson_dslash_amd import F32
+    from sse_wilson_dslash_amd.device import WilsonDslash
+    from sse_wilson_dslash_amd.halo import HaloDslash
+
+    def timed(fn):
+        for _ in range(10):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    out = {}
+    for latt in ((32, 32, 32, 8), LATT):
+        op = WilsonDslash(latt, 4, F32, 18)
+        op.gen_gauge(seed=1)
+        psi = op.gen_spinor(seed=1)
+        res = op.new_spinor(zero=True)
+        h = HaloDslash(op, 0b1000)
+        faces = h.new_faces()
+        t_fused = timed(lambda: op.apply(res, psi, 1, 0))
+        t_int = timed(lambda: h.interior(res, psi, 1, 0))
+        t_split = timed(lambda: h.apply(res, psi, 1, 0, faces))
+        key = "x".join(str(v) for v in latt)
+        out[key] = {"fused_us": round(t_fused, 2), "interior_us": round(t_int, 2), "split_us": round(t_split, 2),
+                    "face_bytes_per_direction": 2 * h.face_bytes[3]}
+        log(f"[gpu] halo split (T partitioned) {key}: fused {t_fused:.1f} us, interior {t_int:.1f} us, "
+            f"pack+interior+exterior {t_split:.1f} us, faces {2 * h.face_bytes[3] / 1024:.0f} KiB")
+        del op, psi, res, faces
+        torch.cuda.empty_cache()
+    return out
+
+
 def copy_leg(gib: float = 2.0, reps: int = 20):
     """Device-to-device copy bandwidth (read + write bytes / time): the practical HBM ceiling."""
     import torch
@@ -358,6 +400,8 @@ def main() -> int:
         extra["f64_sloppy_4d"] = half_leg(n, "f64_sloppy", 18)
     if not args.no_mrhs:
         extra["mrhs"] = mrhs_leg()
+    if not args.no_extra:
+        extra["halo_split"] = halo_leg()
     if not args.no_cg:
         extra["cg_48c96"] = cg_leg()
     if not args.no_extra:

@@ -220,6 +220,43 @@ int dslash_cg_destroy(void* solver);
 int dslash_force(void* d_grad_packed, void* d_force, const void* d_u_packed, const void* d_chi,
                  const void* d_psi, const int latt[4], int isign, int cb, void* stream);
 
+/* ---- domain-decomposition stages (the parscalar split) ------------------ */
+/* The reference's multi-node Dslash (lib/sse_su3dslash_32bit_parscalar.c:108-820,
+ * driven at :1185-1392) splits D into decomp / decomp_hvv (project the source
+ * spinor, and for backward hops multiply by U^dagger at the source site),
+ * an exchange of boundary half-spinors, and mvv_recons / recons on the
+ * target.  These entry points are that split for the directions set in
+ * dir_mask (bit mu = direction mu is partitioned; nd = 3 allows bits 0..2):
+ *   D psi = halo_interior + halo_exterior(exchange(halo_pack)).
+ * Fields are full-lattice device fields in the layout above, d_gauge is the
+ * full gauge object (both target-cb bundles), cb is the SOURCE checkerboard
+ * as in sse_su3dslash_wilson, prec F32 or F64.  On one GPU the exchange is the
+ * identity (pass the send buffer as recv: the periodic wrap); across nodes
+ * recv face (mu, forward) is the +mu neighbour's send face (mu, forward) and
+ * recv face (mu, backward) the -mu neighbour's send face (mu, backward).
+ *
+ * Face buffer: per partitioned mu, the forward face (source sites on x_mu = 0,
+ * projected for the -mu neighbour's forward hops, 12 reals/site) then the
+ * backward face (source sites on x_mu = L_mu - 1, projected and multiplied by
+ * U_mu^dagger, for the +mu neighbour's backward hops); each face holds
+ * Vh / L_mu sites as [12 reals][site] planes.  offsets[2*mu] / offsets[2*mu+1]
+ * give the byte offsets of the forward / backward face of mu, face_bytes[mu]
+ * the size of one face, total_bytes the buffer size (any output may be NULL). */
+int dslash_face_layout(const int latt[4], int nd, int prec, int dir_mask, size_t offsets[8],
+                       size_t face_bytes[4], size_t* total_bytes);
+/* decomp + decomp_hvv of the boundary source sites into d_send. */
+int dslash_halo_pack(void* d_send, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec,
+                     int gauge_reals, int isign, int cb, int dir_mask, void* stream);
+/* res (target cb 1-cb) = every hop of D that stays inside the local lattice
+ * (overwrites, like dslash_apply; dir_mask = 0 is the whole operator). */
+int dslash_halo_interior(void* d_res, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec,
+                         int gauge_reals, int isign, int cb, int dir_mask, void* stream);
+/* res (target cb) += the boundary hops from d_recv: U_mu(x) h for forward
+ * faces (mvv_recons), h for backward faces (recons); directions in increasing
+ * mu, launched in order on the stream. */
+int dslash_halo_exterior(void* d_res, const void* d_recv, const void* d_gauge, const int latt[4], int nd, int prec,
+                         int gauge_reals, int isign, int cb, int dir_mask, void* stream);
+
 /* ---- synthetic fields ------------------------------------------------- */
 /* Fill a legacy-layout gauge [V][4][3][3][2] (prec F32/F64, canonical order of
  * nd) with SU(3) links: complex Gaussian 3x3 reunitarised by Gram-Schmidt,

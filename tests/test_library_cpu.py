@@ -176,6 +176,28 @@ def test_tune_file_round_trip(tmp_path):
     assert lib.dslash_tune_load(b"/nonexistent/tune.txt") == _lib.ERR_INVALID
 
 
+def test_face_layout():
+    """Face buffer of the domain-decomposition stages: per partitioned mu a
+    forward then a backward face of Vh / L_mu half-spinors (12 reals), each
+    direction's pair 256-byte aligned; invalid directions and formats rejected."""
+    lib = _lib.core()
+    latt = (4, 6, 8, 10)
+    vh = 4 * 6 * 8 * 10 // 2
+    off, fb, tot = (C.c_size_t * 8)(), (C.c_size_t * 4)(), C.c_size_t()
+    assert lib.dslash_face_layout(_lib.latt_arg(latt), 4, _lib.F64, 0b1010, off, fb, C.byref(tot)) == 0
+    a256 = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    assert fb[1] == vh // 6 * 12 * 8 and fb[3] == vh // 10 * 12 * 8 and fb[0] == fb[2] == 0
+    assert off[2] == 0 and off[3] == fb[1]
+    assert off[6] == a256(2 * fb[1]) and off[7] == off[6] + fb[3]
+    assert tot.value == off[6] + a256(2 * fb[3])
+    assert lib.dslash_face_layout(_lib.latt_arg(latt), 4, _lib.F32, 0, None, None, C.byref(tot)) == 0
+    assert tot.value == 0
+    assert lib.dslash_face_layout(_lib.latt_arg((4, 4, 4, 3)), 3, _lib.F32, 0b1000, None, None, None) == _lib.ERR_INVALID
+    assert lib.dslash_face_layout(_lib.latt_arg(latt), 4, _lib.F16, 1, None, None, None) == _lib.ERR_INVALID
+    st = lib.dslash_halo_interior(1, 2, 3, _lib.latt_arg(latt), 4, _lib.F32, 18, 1, 0, 16, None)
+    assert st == _lib.ERR_INVALID and b"dir_mask" in lib.dslash_last_error_message()
+
+
 def test_sloppy_switch_host_logic():
     """sse_su3dslash_set_sloppy exists in both libraries; only the FP64 one
     accepts it (no device work is done by the switch itself)."""

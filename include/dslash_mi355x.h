@@ -145,9 +145,18 @@ typedef struct {
  * each link for all of them.  grid_blocks = 0 launches one block per tile;
  * grid_blocks > 0 launches a persistent grid of that many blocks that strides
  * over the tiles of all lattices.  rhs_block / block_threads = 0 take the
- * tuned (or default) values. */
+ * tuned (or default) values; so does the tile shape (t_block below). */
 int dslash_apply_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, int gauge_reals,
                       int rhs_block, int block_threads, int grid_blocks, void* stream);
+/* The batched launch shape used for (prec, gauge_reals, persistent): rhs_block,
+ * block_threads, grid_blocks (0 unless persistent) and t_block: 1 = one site
+ * per thread, 4 = each thread walks 4 consecutive t-slices of one spatial site
+ * and keeps the projected t-neighbour half-spinor for the next-but-one slice
+ * (rhs_block 1 or 2 only).  set: same arguments; the autotuner also sets them. */
+int dslash_get_mrhs_launch(int prec, int gauge_reals, int persistent, int* rhs_block, int* block_threads,
+                           int* grid_blocks, int* t_block);
+int dslash_set_mrhs_launch(int prec, int gauge_reals, int persistent, int rhs_block, int block_threads,
+                           int grid_blocks, int t_block);
 
 /* ---- autotuning -------------------------------------------------------- */
 /* Time every launch shape of the single-RHS kernel (block size x XCD remap) on
@@ -156,8 +165,9 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, 
  * for (nd, prec, gauge_reals) and report it. */
 int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec,
                       int gauge_reals, void* stream, int* block_threads, int* xcd_remap, float* best_ms);
-/* Time rhs_block x block_threads (x persistent grid sizes when persistent != 0)
- * for a batched launch; keeps the fastest for (prec, gauge_reals, persistent). */
+/* Time rhs_block x block_threads x t_block (x persistent grid sizes when
+ * persistent != 0) for a batched launch; keeps the fastest for (prec,
+ * gauge_reals, persistent) (dslash_get_mrhs_launch reports t_block). */
 int dslash_tune_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, int gauge_reals, int persistent,
                      void* stream, int* rhs_block, int* block_threads, int* grid_blocks, float* best_ms);
 /* Tuned choices as text lines; load applies them (missing file = DSLASH_ERR_INVALID). */

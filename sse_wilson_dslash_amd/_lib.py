@@ -68,6 +68,8 @@ PROTOTYPES = {
     "dslash_tune_apply": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _vp, _ip, _ip, _fp]),
     "dslash_tune_mrhs": (_i, [_vp, _i, _i, _i, _i, _vp, _ip, _ip, _ip, _fp]),
     "dslash_force": (_i, [_vp, _vp, _vp, _vp, _vp, _ip, _i, _i, _vp]),
+    "dslash_get_mrhs_launch": (_i, [_i, _i, _i, _ip, _ip, _ip, _ip]),
+    "dslash_set_mrhs_launch": (_i, [_i, _i, _i, _i, _i, _i, _i]),
     "dslash_face_layout": (_i, [_ip, _i, _i, _i, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)]),
     "dslash_halo_pack": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _i, _i, _i, _vp]),
     "dslash_halo_interior": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _i, _i, _i, _vp]),

@@ -82,7 +82,23 @@ def tune_batch(items, persistent: bool = False) -> dict:
     R, B, Gr, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
     check(_lib.core().dslash_tune_mrhs(arr, len(items), prec, G, 1 if persistent else 0, current_stream(),
                                        C.byref(R), C.byref(B), C.byref(Gr), C.byref(ms)))
-    return {"rhs_block": R.value, "block_threads": B.value, "grid_blocks": Gr.value, "ms": ms.value}
+    out = {"rhs_block": R.value, "block_threads": B.value, "grid_blocks": Gr.value, "ms": ms.value}
+    out["t_block"] = get_launch(prec, G, persistent)["t_block"]
+    return out
+
+
+def get_launch(prec: int, G: int, persistent: bool = False) -> dict:
+    """The batched launch shape apply_batch uses for (format, link reals, persistent)."""
+    R, B, Gr, T = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    check(_lib.core().dslash_get_mrhs_launch(prec, G, 1 if persistent else 0, C.byref(R), C.byref(B),
+                                             C.byref(Gr), C.byref(T)))
+    return {"rhs_block": R.value, "block_threads": B.value, "grid_blocks": Gr.value, "t_block": T.value}
+
+
+def set_launch(prec: int, G: int, persistent: bool, rhs_block: int, block_threads: int, grid_blocks: int,
+               t_block: int) -> None:
+    check(_lib.core().dslash_set_mrhs_launch(prec, G, 1 if persistent else 0, rhs_block, block_threads,
+                                             grid_blocks, t_block))
 
 
 def tune_apply(op: WilsonDslash, res: torch.Tensor, psi: torch.Tensor) -> dict:

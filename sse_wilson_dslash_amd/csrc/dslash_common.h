@@ -417,14 +417,16 @@ template <int P, bool NT> __device__ inline void store_spinor(v4u* __restrict__ 
 }
 
 // Load the two links of direction mu for one target site (links are stored in
-// the format link_prec(P) and converted to P's arithmetic type).
-template <int P, int G> __device__ inline void load_pair(const v4u* __restrict__ pair_base, uint32_t idx, uint32_t vh, C<typename Fmt<P>::compute> Wf[3][3], C<typename Fmt<P>::compute> Wb[3][3]) {
+// the format link_prec(P) and converted to P's arithmetic type).  NT: the
+// links are read once per launch (non-temporal loads, the single-RHS kernel);
+// kernels that reuse a link from L2 (batched right-hand sides) load normally.
+template <int P, int G, bool NT = true> __device__ inline void load_pair(const v4u* __restrict__ pair_base, uint32_t idx, uint32_t vh, C<typename Fmt<P>::compute> Wf[3][3], C<typename Fmt<P>::compute> Wb[3][3]) {
   constexpr int PL = LinkP<P>::value;
   typedef typename Fmt<PL>::real R;
   constexpr int NV = PairVec<P, G>::value;
   v4u buf[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) buf[v] = ld_nt(&pair_base[(size_t)v * vh + idx]);
+  for (int v = 0; v < NV; ++v) buf[v] = NT ? ld_nt(&pair_base[(size_t)v * vh + idx]) : pair_base[(size_t)v * vh + idx];
   const R* r = reinterpret_cast<const R*>(buf);
   constexpr int ROWS = G / 6;
 #pragma unroll

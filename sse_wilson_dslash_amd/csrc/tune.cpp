@@ -4,7 +4,7 @@
 // stream, on the caller's own fields, and keeps the fastest per kernel family:
 //   single-RHS Dslash:  (nd, format, link reals) -> threads per block, XCD remap
 //   batched Dslash:     (format, link reals, persistent) -> RHS block, threads
-//                        per block, persistent grid size
+//                        per block, persistent grid size, t-slices per thread
 // Choices can be saved to and reloaded from a small text file, so a run tunes
 // once per machine and later processes start tuned.
 #include <hip/hip_runtime.h>
@@ -24,22 +24,24 @@ struct MrhsCfg {
   int R;
   int block;
   int grid;  // persistent grid blocks (0 for the one-tile-per-block launch)
+  int tb;    // t-slices per thread (1, or MRHS_T_BLOCK for t-marching tiles)
 };
 
-// [format F32/F64][G==12][persistent]
-static MrhsCfg g_mrhs[2][2][2] = {{{{2, 128, 0}, {2, 128, 2048}}, {{2, 128, 0}, {2, 128, 2048}}},
-                                  {{{2, 128, 0}, {2, 128, 2048}}, {{2, 128, 0}, {2, 128, 2048}}}};
+// [format F32/F64][G==12][persistent]; defaults measured at 32 RHS on 24^3 x 48
+static MrhsCfg g_mrhs[2][2][2] = {
+    {{{2, 64, 0, MRHS_T_BLOCK}, {2, 64, 4096, MRHS_T_BLOCK}}, {{2, 64, 0, MRHS_T_BLOCK}, {2, 64, 4096, MRHS_T_BLOCK}}},
+    {{{2, 128, 0, 1}, {2, 128, 2048, 1}}, {{2, 128, 0, 1}, {2, 128, 2048, 1}}}};
 
 static MrhsCfg& mrhs_entry(int prec, int G, int persistent) {
   return g_mrhs[prec == F64][G == 12][persistent ? 1 : 0];
 }
 
 static int mrhs_run(const dslash_mrhs_problem* probs, int nprob, int prec, int G, int R, int block, int grid,
-                    hipStream_t s) {
+                    int tb, hipStream_t s) {
   MrhsList list;
-  if (int e = mrhs_build_list(probs, nprob, prec, R, block, &list)) return e;
+  if (int e = mrhs_build_list(probs, nprob, prec, R, block, tb, &list)) return e;
   uint32_t g = grid > 0 ? (uint32_t)grid : list.total_tiles;
-  return mrhs_launch(list, prec, G, R, block, g, s);
+  return mrhs_launch(list, prec, G, R, block, tb, g, s);
 }
 
 template <typename F>
@@ -77,7 +79,36 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int
   const MrhsCfg& c = mrhs_entry(prec, G, grid_blocks > 0);
   int R = rhs_block ? rhs_block : c.R;
   int block = block_threads ? block_threads : c.block;
-  return mrhs_run(probs, nprob, prec, G, R, block, grid_blocks, (hipStream_t)stream);
+  int tb = R > 2 ? 1 : c.tb;  // t-marching tiles exist for rhs_block 1 and 2
+  return mrhs_run(probs, nprob, prec, G, R, block, grid_blocks, tb, (hipStream_t)stream);
+}
+
+int dslash_get_mrhs_launch(int prec, int G, int persistent, int* rhs_block, int* block_threads, int* grid_blocks,
+                           int* t_block) {
+  clear_error();
+  WD_REQUIRE(prec == F32 || prec == F64, "multi-RHS Dslash is built for F32 and F64");
+  WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
+  const MrhsCfg& c = mrhs_entry(prec, G, persistent);
+  if (rhs_block) *rhs_block = c.R;
+  if (block_threads) *block_threads = c.block;
+  if (grid_blocks) *grid_blocks = c.grid;
+  if (t_block) *t_block = c.tb;
+  return 0;
+}
+
+int dslash_set_mrhs_launch(int prec, int G, int persistent, int rhs_block, int block_threads, int grid_blocks,
+                           int t_block) {
+  clear_error();
+  WD_REQUIRE(prec == F32 || prec == F64, "multi-RHS Dslash is built for F32 and F64");
+  WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
+  WD_REQUIRE(rhs_block == 1 || rhs_block == 2 || rhs_block == 4, "rhs_block must be 1, 2 or 4");
+  WD_REQUIRE(block_threads >= 64 && block_threads <= 256 && block_threads % 64 == 0,
+             "block_threads must be 64, 128, 192 or 256");
+  WD_REQUIRE(t_block == 1 || (t_block == MRHS_T_BLOCK && rhs_block <= 2),
+             "t_block must be 1, or %d with rhs_block 1 or 2", MRHS_T_BLOCK);
+  WD_REQUIRE(persistent ? grid_blocks > 0 : grid_blocks == 0, "grid_blocks must be > 0 exactly when persistent");
+  mrhs_entry(prec, G, persistent) = MrhsCfg{rhs_block, block_threads, grid_blocks, t_block};
+  return 0;
 }
 
 int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const int latt[4], int nd, int prec, int G,
@@ -133,24 +164,27 @@ int dslash_tune_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int 
   WD_REQUIRE(prec == F32 || prec == F64, "multi-RHS Dslash is built for F32 and F64");
   WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
   hipStream_t s = (hipStream_t)stream;
-  MrhsCfg best{0, 0, 0};
+  MrhsCfg best{0, 0, 0, 1};
   float best_t = 1e30f;
   const int grids_p[] = {512, 1024, 2048, 4096};
   const int grids_np[] = {0};
   const int* grids = persistent ? grids_p : grids_np;
   const int ng = persistent ? 4 : 1;
-  for (int R : {1, 2, 4})
-    for (int block : {64, 128, 256})
-      for (int gi = 0; gi < ng; ++gi) {
-        float t = 0;
-        int grid = grids[gi];
-        int e = time_it(s, 10, [&]() { return mrhs_run(probs, nprob, prec, G, R, block, grid, s); }, &t);
-        if (e) return e;
-        if (t < best_t) {
-          best_t = t;
-          best = MrhsCfg{R, block, grid};
+  for (int tb : {1, MRHS_T_BLOCK})
+    for (int R : {1, 2, 4}) {
+      if (tb > 1 && R > 2) continue;
+      for (int block : {64, 128, 256})
+        for (int gi = 0; gi < ng; ++gi) {
+          float t = 0;
+          int grid = grids[gi];
+          int e = time_it(s, 10, [&]() { return mrhs_run(probs, nprob, prec, G, R, block, grid, tb, s); }, &t);
+          if (e) return e;
+          if (t < best_t) {
+            best_t = t;
+            best = MrhsCfg{R, block, grid, tb};
+          }
         }
-      }
+    }
   mrhs_entry(prec, G, persistent) = best;
   if (rhs_block) *rhs_block = best.R;
   if (block_threads) *block_threads = best.block;
@@ -175,7 +209,7 @@ int dslash_tune_save(const char* path) {
     for (int G : {18, 12})
       for (int per : {0, 1}) {
         MrhsCfg c = mrhs_entry(p, G, per);
-        fprintf(f, "mrhs %d %d %d %d %d %d\n", p, G, per, c.R, c.block, c.grid);
+        fprintf(f, "mrhs %d %d %d %d %d %d %d\n", p, G, per, c.R, c.block, c.grid, c.tb);
       }
   fclose(f);
   return 0;
@@ -189,7 +223,7 @@ int dslash_tune_load(const char* path) {
   char line[256];
   int bad = 0;
   while (fgets(line, sizeof(line), f)) {
-    int a[6];
+    int a[7];
     if (line[0] == '#') continue;
     if (sscanf(line, "apply %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4]) == 5) {
       if ((a[0] == 3 || a[0] == 4) && a[1] >= 0 && a[1] < NUM_PREC && (a[2] == 18 || a[2] == 12) && a[3] >= 64 &&
@@ -197,10 +231,13 @@ int dslash_tune_load(const char* path) {
         set_launch_cfg(a[0], a[1], a[2], LaunchCfg{a[3], a[4] ? 1 : 0});
       else
         ++bad;
-    } else if (sscanf(line, "mrhs %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]) == 6) {
+    } else if (int n = sscanf(line, "mrhs %d %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &a[6]);
+               n >= 6) {
+      const int tb = n == 7 ? a[6] : 1;  // files written before t-marching tiles carry 6 fields
       if ((a[0] == F32 || a[0] == F64) && (a[1] == 18 || a[1] == 12) && (a[3] == 1 || a[3] == 2 || a[3] == 4) &&
-          a[4] >= 64 && a[4] <= 256 && a[4] % 64 == 0 && a[5] >= 0)
-        mrhs_entry(a[0], a[1], a[2]) = MrhsCfg{a[3], a[4], a[2] ? (a[5] > 0 ? a[5] : 2048) : 0};
+          a[4] >= 64 && a[4] <= 256 && a[4] % 64 == 0 && a[5] >= 0 &&
+          (tb == 1 || (tb == MRHS_T_BLOCK && a[3] <= 2)))
+        mrhs_entry(a[0], a[1], a[2]) = MrhsCfg{a[3], a[4], a[2] ? (a[5] > 0 ? a[5] : 2048) : 0, tb};
       else
         ++bad;
     } else if (line[0] != '\n') {

@@ -6,7 +6,8 @@ import pytest
 import torch
 
 from sse_wilson_dslash_amd import F32, F64
-from sse_wilson_dslash_amd.batch import BatchItem, RhsBatch, apply_batch, gen_batch, tune_apply, tune_batch
+from sse_wilson_dslash_amd.batch import (BatchItem, RhsBatch, apply_batch, gen_batch, get_launch, set_launch,
+                                        tune_apply, tune_batch)
 from sse_wilson_dslash_amd.device import WilsonDslash, gen_gauge_host
 from sse_wilson_dslash_amd.layout import volume
 from tests import refimpl as R
@@ -42,11 +43,23 @@ def check_item(oracle, item, u, host, tol, ks=None):
         assert e < tol, (op.latt, k, e)
 
 
-@pytest.mark.parametrize("rhs_block", [1, 2, 4])
+@pytest.fixture
+def launch_table():
+    """Restore the batched launch table after a test that sets it."""
+    saved = {(p, g, per): get_launch(p, g, per) for p in (F32, F64) for g in (18, 12) for per in (False, True)}
+    yield
+    for (p, g, per), c in saved.items():
+        set_launch(p, g, per, c["rhs_block"], c["block_threads"], c["grid_blocks"], c["t_block"])
+
+
+@pytest.mark.parametrize("rhs_block,t_block", [(1, 1), (2, 1), (4, 1), (1, 4), (2, 4)])
 @pytest.mark.parametrize("block", [64, 256])
-def test_mrhs_matches_single_rhs_kernel(oracle, rhs_block, block):
-    latt = (8, 8, 8, 16)
+@pytest.mark.parametrize("latt", [(8, 8, 8, 16), (8, 4, 6, 10)])
+def test_mrhs_matches_single_rhs_kernel(oracle, launch_table, rhs_block, t_block, block, latt):
+    """Both tile shapes (one site per thread; t-marching over 4 slices, with a
+    partial last chunk when T = 10) against the single-RHS kernel."""
     item, u, host = make_item(latt, 7, isign=-1, target_cb=1)  # 7: exercises a partial RHS block
+    set_launch(F32, 18, False, rhs_block, block, 0, t_block)
     apply_batch([item], rhs_block=rhs_block, block_threads=block)
     torch.cuda.synchronize()
     op = item.op
@@ -78,10 +91,12 @@ def uneven_items():
 
 
 @pytest.mark.parametrize("grid", [0, 8, 13, 512, 2048])
-def test_persistent_uneven_lattices(oracle, grid):
+@pytest.mark.parametrize("t_block", [1, 4])
+def test_persistent_uneven_lattices(oracle, launch_table, grid, t_block):
     """Config: one launch over 8^4, 16^3x32 and 24^3x48 with 4 RHS each."""
-    print(f"[persistent] grid={grid}", flush=True)
+    print(f"[persistent] grid={grid} t_block={t_block}", flush=True)
     items = uneven_items()
+    set_launch(F32, 18, grid > 0, 2, 128, grid, t_block)
     apply_batch([it[0] for it in items], rhs_block=2, block_threads=128, grid_blocks=grid)
     torch.cuda.synchronize()
     for item, u, host in items:
@@ -131,6 +146,7 @@ def test_autotuner(oracle):
     for persistent in (False, True):
         r = tune_batch(its, persistent)
         assert r["rhs_block"] in (1, 2, 4) and r["block_threads"] in (64, 128, 256) and r["ms"] > 0
+        assert r["t_block"] in (1, 4) and not (r["t_block"] == 4 and r["rhs_block"] == 4)
         assert (r["grid_blocks"] > 0) == persistent
         for it in its:
             it.res.buf.zero_()

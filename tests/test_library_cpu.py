@@ -176,6 +176,30 @@ def test_tune_file_round_trip(tmp_path):
     assert lib.dslash_tune_load(b"/nonexistent/tune.txt") == _lib.ERR_INVALID
 
 
+def test_mrhs_launch_table_and_tune_file(tmp_path):
+    lib = _lib.core()
+    v = [C.c_int() for _ in range(4)]
+    assert lib.dslash_get_mrhs_launch(_lib.F32, 18, 0, *[C.byref(x) for x in v]) == 0
+    saved = [x.value for x in v]
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 4, 128, 0, 4) == _lib.ERR_INVALID  # no R=4 marching
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 128, 512, 1) == _lib.ERR_INVALID  # grid only if persistent
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 192, 0, 3) == _lib.ERR_INVALID
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 192, 0, 4) == 0
+    path = str(tmp_path / "tune.txt")
+    assert lib.dslash_tune_save(path.encode()) == 0
+    assert "mrhs 0 18 0 1 192 0 4\n" in open(path).read()
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, *saved) == 0
+    assert lib.dslash_tune_load(path.encode()) == 0
+    lib.dslash_get_mrhs_launch(_lib.F32, 18, 0, *[C.byref(x) for x in v])
+    assert [x.value for x in v] == [1, 192, 0, 4]
+    old = tmp_path / "old.txt"  # a file from before t-marching tiles: 6 fields, t_block 1
+    old.write_text("mrhs 0 18 0 2 128 0\n")
+    assert lib.dslash_tune_load(str(old).encode()) == 0
+    lib.dslash_get_mrhs_launch(_lib.F32, 18, 0, *[C.byref(x) for x in v])
+    assert [x.value for x in v] == [2, 128, 0, 1]
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, *saved) == 0
+
+
 def test_face_layout():
     """Face buffer of the domain-decomposition stages: per partitioned mu a
     forward then a backward face of Vh / L_mu half-spinors (12 reals), each

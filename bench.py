@@ -199,6 +199,51 @@ def halo_leg(reps: int = 200):
     return out
 
 
+def force_leg(reps: int = 50):
+    """Backward kernel (link gradient and su(3) force of Re<chi, D psi>, FP64) at
+    16^3x32 (the backward config) and 32^3x64.  Algorithmic bytes per link: the
+    two spinor fields read once (2 x 192 B per site pair = 48 B per link), the link
+    read once (144 B), both outputs written once (2 x 144 B)."""
+    import torch
+    from sse_wilson_dslash_amd import F64
+    from sse_wilson_dslash_amd._lib import check, latt_arg, ptr
+    from sse_wilson_dslash_amd.device import WilsonDslash, current_stream
+
+    out = {}
+    for latt in ((16, 16, 16, 32), LATT):
+        op = WilsonDslash(latt, 4, F64, 18)
+        d_u = torch.empty(op.V * 72, dtype=torch.float64, device="cuda")
+        check(op.lib.dslash_gen_gauge_device(ptr(d_u), F64, latt_arg(latt), 4, 1, current_stream()))
+        chi = op.gen_spinor(seed=2, field=0)
+        psi = op.gen_spinor(seed=2, field=1)
+        grad = torch.empty_like(d_u)
+        force = torch.empty_like(d_u)
+
+        def call(k):
+            check(op.lib.dslash_force(ptr(grad), ptr(force), ptr(d_u), ptr(chi), ptr(psi), latt_arg(latt),
+                                      1 if k % 2 == 0 else -1, k % 2, current_stream()))
+
+        for k in range(5):
+            call(k)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(reps):
+            call(k)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        links = 4 * op.V
+        gbs = links * 480 / (ms * 1e-3) / 1e9
+        key = "x".join(str(v) for v in latt)
+        out[key] = {"us_per_call": round(ms * 1e3, 2), "links": links, "hbm_gbs": round(gbs, 1),
+                    "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
+        log(f"[force] {key} FP64 gradient + su(3) force: {ms * 1e3:.1f} us/call, {gbs:.0f} GB/s "
+            f"({gbs / PEAK_HBM_GBS:.1%} of HBM peak, 480 B/link)")
+        del op, d_u, chi, psi, grad, force
+        torch.cuda.empty_cache()
+    return out
+
+
 def copy_leg(gib: float = 2.0, reps: int = 20):
     """Device-to-device copy bandwidth (read + write bytes / time): the practical HBM ceiling."""
     import torch
@@ -403,6 +448,7 @@ def main() -> int:
         extra["mrhs"] = mrhs_leg()
     if not args.no_extra:
         extra["halo_split"] = halo_leg()
+        extra["force"] = force_leg()
     if not args.no_cg:
         extra["cg_48c96"] = cg_leg()
     if not args.no_extra:

@@ -226,6 +226,7 @@ int dslash_cg_destroy(void* solver);
  *   d_force        the su(3) projection F = TA((U G)^dagger) (row-major), with
  *                  dS = Re tr(X^dagger F) for dU = X U, X in su(3); needs
  *                  d_u_packed (F64 packed links, canonical order).
+ * The three link arrays must be 16-byte aligned (as any hipMalloc pointer is).
  * The gradient with respect to psi is D^dagger chi (dslash_apply with -isign). */
 int dslash_force(void* d_grad_packed, void* d_force, const void* d_u_packed, const void* d_chi,
                  const void* d_psi, const int latt[4], int isign, int cb, void* stream);

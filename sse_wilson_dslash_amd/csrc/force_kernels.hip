@@ -14,7 +14,9 @@
 //   * the su(3) force F = TA((U G)^dagger), TA(M) = (M - M^dagger)/2 - tr(M - M^dagger)/6,
 //     the unique traceless anti-Hermitian matrix with dS = Re tr(X^dagger F)
 //     for dU = X U, X in su(3).
-// One thread per (site, direction); FP64.
+// One thread per (site, direction); FP64.  A block owns 64 sites and their four
+// directions, so its output records are one contiguous 36 KiB range that it
+// writes with coalesced vector stores (see k_force).
 #include <hip/hip_runtime.h>
 
 #include "dslash_common.h"
@@ -60,69 +62,117 @@ __device__ inline void force_link(const Geom& g, const v4u* chi_t, const float* 
     }
 }
 
+// Block = 64 sites x 4 directions (wave w = direction w, lane = site), i.e. the
+// 256 consecutive 144-byte records [4*site + mu] of the output arrays.  Link
+// records are read and output records written cooperatively through a 36 KiB
+// LDS tile with fully coalesced 16-byte accesses; each thread works on its own
+// record in LDS.
+constexpr int FORCE_SITES = 64;
+constexpr int FORCE_REC = 4 * FORCE_SITES;  // records per block
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// Cooperative copy of the block's records between global memory and the tile
+// (9 16-byte vectors per record; nrec records, the last block may be partial).
+template <bool TO_GLOBAL>
+__device__ inline void tile_copy(d2* tile, d2* glob, uint32_t nrec) {
+  const uint32_t nvec = nrec * 9;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const uint32_t c = (uint32_t)k * FORCE_REC + threadIdx.x;
+    if (c < nvec) {
+      if (TO_GLOBAL) glob[c] = tile[c];
+      else tile[c] = glob[c];
+    }
+  }
+}
+
 template <int IS>
-__global__ void __launch_bounds__(256) k_force(double* __restrict__ grad, double* __restrict__ force,
-                                               const double* __restrict__ u, const v4u* chi, const v4u* psi,
-                                               size_t cb_vec, size_t norm_off, Geom g, int tcb) {
-  const uint32_t full = blockIdx.x * blockDim.x + threadIdx.x;
-  const int mu = (int)blockIdx.y;
-  if (full >= 2 * g.vh) return;
-  const int ycb = full >= g.vh ? 1 : 0;
-  const uint32_t idx = full - (uint32_t)ycb * g.vh;
-  const Coord c = site_coords<4>(g, idx, ycb);
-  const int scb = 1 - tcb;
-  const v4u* chi_t = chi + (size_t)tcb * cb_vec;
-  const v4u* psi_s = psi + (size_t)scb * cb_vec;
-  const float* chi_tn = (const float*)(chi_t + norm_off);
-  const float* psi_sn = (const float*)(psi_s + norm_off);
+__global__ void __launch_bounds__(FORCE_REC) k_force(double* __restrict__ grad, double* __restrict__ force,
+                                                     const double* __restrict__ u, const v4u* chi, const v4u* psi,
+                                                     size_t cb_vec, size_t norm_off, Geom g, int tcb) {
+  __shared__ d2 tile[FORCE_REC * 9];
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int mu = (int)(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t full0 = blk * FORCE_SITES;
+  const uint32_t nsite = min((uint32_t)FORCE_SITES, 2 * g.vh - full0);
+  const uint32_t nrec = 4 * nsite;
+  const uint32_t full = full0 + lane;
+  const bool live = lane < nsite;
+  const size_t rec0 = (size_t)full0 * 4 * 9;  // first vector of the block's records
+  const uint32_t my = (lane * 4 + (uint32_t)mu) * 9;  // this thread's record in the tile
+
   C<double> G[3][3];
-  switch (mu) {
-    case 0: force_link<0, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
-    case 1: force_link<1, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
-    case 2: force_link<2, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
-    default: force_link<3, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
+  if (live) {
+    const int ycb = full >= g.vh ? 1 : 0;
+    const uint32_t idx = full - (uint32_t)ycb * g.vh;
+    const Coord c = site_coords<4>(g, idx, ycb);
+    const int scb = 1 - tcb;
+    const v4u* chi_t = chi + (size_t)tcb * cb_vec;
+    const v4u* psi_s = psi + (size_t)scb * cb_vec;
+    const float* chi_tn = (const float*)(chi_t + norm_off);
+    const float* psi_sn = (const float*)(psi_s + norm_off);
+    switch (mu) {
+      case 0: force_link<0, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
+      case 1: force_link<1, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
+      case 2: force_link<2, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
+      default: force_link<3, IS>(g, chi_t, chi_tn, psi_s, psi_sn, tcb, ycb, idx, c, G); break;
+    }
   }
-  const size_t base = ((size_t)full * 4 + mu) * 18;
-  if (grad) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        grad[base + (a * 3 + b) * 2] = G[a][b].re;
-        grad[base + (a * 3 + b) * 2 + 1] = -G[a][b].im;
-      }
-  }
+  d2 F[9];
   if (force) {
-    C<double> U[3][3], A[3][3];
+    tile_copy<false>(tile, const_cast<d2*>(reinterpret_cast<const d2*>(u)) + rec0, nrec);
+    __syncthreads();
+    if (live) {
+      C<double> U[3][3], A[3][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) U[i][j] = {u[base + (j * 3 + i) * 2], u[base + (j * 3 + i) * 2 + 1]};
+        for (int j = 0; j < 3; ++j) {
+          const d2 e = tile[my + j * 3 + i];  // stored transposed: U(i,j) = stored[j][i]
+          U[i][j] = {e.x, e.y};
+        }
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        C<double> acc{0.0, 0.0};
+        for (int j = 0; j < 3; ++j) {
+          C<double> acc{0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < 3; ++k) cmac(acc, U[i][k], G[k][j]);
-        A[i][j] = acc;
-      }
-    // M = A^dagger ; F = (M - M^dagger)/2 - tr(M - M^dagger)/6
-    double trim = 0.0;  // Im tr(M - M^dagger) / 2 = -Im tr(A)
+          for (int k = 0; k < 3; ++k) cmac(acc, U[i][k], G[k][j]);
+          A[i][j] = acc;
+        }
+      // M = A^dagger ; F = (M - M^dagger)/2 - tr(M - M^dagger)/6
+      double trim = 0.0;  // Im tr(M - M^dagger) / 2 = -Im tr(A)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) trim += -A[i][i].im;
-    trim /= 3.0;
+      for (int i = 0; i < 3; ++i) trim += -A[i][i].im;
+      trim /= 3.0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        // M_ij = conj(A_ji); (M - M^dag)_ij / 2 = (conj(A_ji) - A_ij) / 2
-        double re = 0.5 * (A[j][i].re - A[i][j].re);
-        double im = 0.5 * (-A[j][i].im - A[i][j].im);
-        if (i == j) im -= trim;
-        force[base + (i * 3 + j) * 2] = re;
-        force[base + (i * 3 + j) * 2 + 1] = im;
-      }
+        for (int j = 0; j < 3; ++j) {
+          // M_ij = conj(A_ji); (M - M^dag)_ij / 2 = (conj(A_ji) - A_ij) / 2
+          double re = 0.5 * (A[j][i].re - A[i][j].re);
+          double im = 0.5 * (-A[j][i].im - A[i][j].im);
+          if (i == j) im -= trim;
+          F[i * 3 + j] = d2{re, im};
+        }
+    }
+    __syncthreads();  // every thread has read its link record
+    if (live)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) tile[my + k] = F[k];
+    __syncthreads();
+    tile_copy<true>(tile, reinterpret_cast<d2*>(force) + rec0, nrec);
+  }
+  if (grad) {
+    if (force) __syncthreads();  // the force tile has been stored
+    if (live)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) tile[my + a * 3 + b] = d2{G[a][b].re, -G[a][b].im};
+    __syncthreads();
+    tile_copy<true>(tile, reinterpret_cast<d2*>(grad) + rec0, nrec);
   }
 }
 
@@ -138,9 +188,11 @@ extern "C" int dslash_force(void* d_grad_packed, void* d_force, const void* d_u_
   WD_REQUIRE(cb == 0 || cb == 1, "cb (source checkerboard) must be 0 or 1");
   WD_REQUIRE(d_chi && d_psi && (d_grad_packed || d_force), "NULL pointer");
   WD_REQUIRE(!d_force || d_u_packed, "the su(3) force needs the links");
+  WD_REQUIRE((((uintptr_t)d_grad_packed | (uintptr_t)d_force | (uintptr_t)d_u_packed) & 15) == 0,
+             "link arrays must be 16-byte aligned");
   Geom g = make_geom(latt, 4);
   const size_t cb_bytes = spinor_cb_bytes(g.vh, F64);
-  dim3 grid((2 * g.vh + 255) / 256, 4), blk(256);
+  dim3 grid((2 * g.vh + FORCE_SITES - 1) / FORCE_SITES), blk(FORCE_REC);
   hipStream_t s = (hipStream_t)stream;
   const size_t cb_vec = cb_bytes / 16, norm_off = (size_t)spinor_vec(F64) * g.vh;
   if (isign > 0)

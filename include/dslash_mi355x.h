@@ -149,10 +149,15 @@ typedef struct {
 int dslash_apply_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, int gauge_reals,
                       int rhs_block, int block_threads, int grid_blocks, void* stream);
 /* The batched launch shape used for (prec, gauge_reals, persistent): rhs_block,
- * block_threads, grid_blocks (0 unless persistent) and t_block: 1 = one site
- * per thread, 4 = each thread walks 4 consecutive t-slices of one spatial site
- * and keeps the projected t-neighbour half-spinor for the next-but-one slice
- * (rhs_block 1 or 2 only).  set: same arguments; the autotuner also sets them. */
+ * block_threads, grid_blocks (0 unless persistent) and t_block, the tile shape:
+ *   1  one site per thread (any rhs_block);
+ *   4  each thread walks 4 consecutive t-slices of one spatial site and keeps the
+ *      projected t-neighbour half-spinor for the next-but-one slice (rhs_block 1, 2);
+ *   8  a 128-thread block walks 8 slices of 128 spatial sites together and stages
+ *      each slice's spinors in a double-buffered LDS slab for the x and y hops
+ *      (F32, rhs_block 1, 2, block_threads 128; launches whose problems share one
+ *      isign and have nrhs divisible by rhs_block; others fall back to 4).
+ * set: same arguments; the autotuner also sets them. */
 int dslash_get_mrhs_launch(int prec, int gauge_reals, int persistent, int* rhs_block, int* block_threads,
                            int* grid_blocks, int* t_block);
 int dslash_set_mrhs_launch(int prec, int gauge_reals, int persistent, int rhs_block, int block_threads,

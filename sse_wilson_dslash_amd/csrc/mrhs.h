@@ -10,7 +10,21 @@
 namespace wd {
 
 constexpr int MRHS_MAX_LATTICES = 8;
-constexpr int MRHS_T_BLOCK = 4;  // t-slices per thread in t-marching tiles
+constexpr int MRHS_T_BLOCK = 4;      // t-slices per thread in t-marching tiles
+constexpr int MRHS_T_BLOCK_LDS = 8;  // t-slices per block in LDS-staged t-marching tiles
+constexpr int MRHS_LDS_THREADS = 128;  // threads (= spatial sites) per LDS-staged block
+constexpr int MRHS_LDS_GROUP = 8;    // RHS blocks of one spatial block run together
+
+// Tile shapes: t_block 1 (one site per thread, any rhs_block), MRHS_T_BLOCK
+// (t-marching, rhs_block <= 2), MRHS_T_BLOCK_LDS (t-marching with an LDS slab,
+// F32, rhs_block <= 2, MRHS_LDS_THREADS threads).
+inline bool mrhs_shape_ok(int prec, int R, int block, int t_block) {
+  if (!(R == 1 || R == 2 || R == 4) || block < 64 || block > 256 || block % 64) return false;
+  if (t_block == 1) return true;
+  if (t_block == MRHS_T_BLOCK) return R <= 2;
+  if (t_block == MRHS_T_BLOCK_LDS) return prec == F32 && R <= 2 && block == MRHS_LDS_THREADS;
+  return false;
+}
 
 struct MrhsLattice {
   Geom g;
@@ -25,6 +39,7 @@ struct MrhsLattice {
   uint32_t nsb;      // site blocks (t_block = 1) or spatial-site blocks (t_block > 1)
   uint32_t nrb;      // rhs blocks
   uint32_t ntc;      // t-chunks (t_block > 1)
+  uint32_t rg;       // RHS blocks per group (LDS-staged tiles; nrb otherwise)
   uint32_t S;        // target sites per t-slice (Lx/2 * Ly * Lz)
   FastDiv dlxh, dly; // for the spatial coordinates of a t-slice site
 };
@@ -40,6 +55,12 @@ struct MrhsList {
 // thread walks the chunk's slices.
 int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R, int block, int t_block,
                     MrhsList* list);
+// LDS-staged tiles also need every problem to have nrhs % R == 0 and one isign.
+inline bool mrhs_lds_compatible(const dslash_mrhs_problem* probs, int nprob, int R) {
+  for (int i = 0; i < nprob; ++i)
+    if (probs[i].nrhs % R != 0 || probs[i].isign != probs[0].isign) return false;
+  return nprob >= 1;
+}
 int mrhs_launch(const MrhsList& list, int prec, int G, int R, int block, int t_block, uint32_t grid, hipStream_t s);
 
 }  // namespace wd

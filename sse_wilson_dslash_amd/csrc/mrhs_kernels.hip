@@ -19,6 +19,17 @@
 // its projected half-spinor is kept in registers for that hop: with 32 fields
 // in flight the t neighbours (Lx/2*Ly*Lz sites apart) no longer fit L2 between
 // their two uses, and this halves the t-direction reloads.
+//
+// LDS-staged t-marching tiles (t_block 8, FP32): a 128-thread block owns 128
+// consecutive spatial sites of one (t-chunk, RHS block) and walks 8 slices
+// together.  At each step every thread loads the spinor psi(t+1) at its own
+// spatial site (its forward t neighbour) and stores it into a double-buffered
+// LDS slab; one step later the slab is slice t+1 of the source, and the x and y
+// hops of the whole block read their neighbours from it (neighbours outside the
+// block, and all z hops, load from global memory; a generic pointer selects LDS
+// or global per lane without a divergent branch).  Tiles are ordered (t-chunk,
+// group of 8 RHS blocks, spatial block, RHS block in group) so the blocks that
+// exchange z neighbours run at the same time.
 #include <hip/hip_runtime.h>
 
 #include <string.h>
@@ -139,6 +150,157 @@ __device__ __forceinline__ void mrhs_march(const MrhsLattice& L, uint32_t s, int
   }
 }
 
+// FP32 spinor from its six 16-byte vectors.
+__device__ inline Spinor<float> spinor_of(const v4u (&buf)[6]) {
+  const float* fr = reinterpret_cast<const float*>(buf);
+  Spinor<float> q;
+#pragma unroll
+  for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) q.s[sp][cc] = {fr[(sp * 3 + cc) * 2], fr[(sp * 3 + cc) * 2 + 1]};
+  return q;
+}
+
+// One spinor through a generic pointer: an LDS slab (stride = block) or global
+// planes (stride = vh), FP32.
+__device__ inline Spinor<float> spinor_at(const v4u* p, size_t stride) {
+  v4u buf[6];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) buf[v] = p[(size_t)v * stride];
+  return spinor_of(buf);
+}
+
+template <int G, int IS, int R, int MU>
+__device__ __forceinline__ void lds_hop(const MrhsLattice& L, v4u (*slab)[6][MRHS_LDS_THREADS], const Coord& c,
+                                        uint32_t idx, uint32_t s0, uint32_t tS, int r0,
+                                        Spinor<float> (&acc)[R]) {
+  constexpr int BS = MRHS_LDS_THREADS;
+  constexpr int PV = PairVec<F32, G>::value;
+  const uint32_t vh = L.g.vh;
+  C<float> Wf[3][3], Wb[3][3];
+  load_pair<F32, G, false>(L.gauge + (size_t)MU * PV * vh, idx, vh, Wf, Wb);
+  // spatial indices of the two neighbours within slice t
+  const uint32_t nf = site_index<4>(L.g, shift_coord(L.g, c, MU, +1)) - tS;
+  const uint32_t nb = site_index<4>(L.g, shift_coord(L.g, c, MU, -1)) - tS;
+  const bool inf = MU < 2 && nf - s0 < (uint32_t)BS, inb = MU < 2 && nb - s0 < (uint32_t)BS;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + tS;
+    const v4u* pf = inf ? &slab[r][0][nf - s0] : f + nf;
+    const v4u* pb = inb ? &slab[r][0][nb - s0] : f + nb;
+    const Spinor<float> p = spinor_at(pf, inf ? (size_t)BS : (size_t)vh);
+    const Spinor<float> q = spinor_at(pb, inb ? (size_t)BS : (size_t)vh);
+    spin_recon_add<MU, -IS>(acc[r], su3_mul(Wf, spin_project<MU, -IS>(p)));
+    spin_recon_add<MU, IS>(acc[r], su3_mul(Wb, spin_project<MU, IS>(q)));
+  }
+}
+
+// One LDS-staged tile: spatial sites [s0, s0 + BS) of slices t0 .. t0+7 (clipped
+// to T), fields r0 .. r0+R-1 (LDS tiles need nrhs to be a multiple of R, so the
+// RHS block is always full).  Every thread of the block executes the same
+// barriers: the step loop bound and the tile are uniform over the block.
+template <int G, int IS, int R>
+__device__ void mrhs_march_lds(const MrhsLattice& L, v4u (*slab)[R][6][MRHS_LDS_THREADS], uint32_t s0, int t0,
+                               int r0) {
+  constexpr int BS = MRHS_LDS_THREADS;
+  constexpr int TB = MRHS_T_BLOCK_LDS;
+  constexpr int PV = PairVec<F32, G>::value;
+  const uint32_t vh = L.g.vh, S = L.S;
+  const int LT = L.g.L[3];
+  const uint32_t s = s0 + threadIdx.x;
+  const bool live = s < S;
+  const uint32_t sc = live ? s : S - 1;
+  const uint32_t q0 = fdiv(sc, L.dlxh);
+  const int xh = (int)(sc - q0 * (uint32_t)L.g.lxh);
+  const uint32_t q1 = fdiv(q0, L.dly);
+  const int y = (int)(q0 - q1 * (uint32_t)L.g.L[1]), z = (int)q1;
+  // preamble: slice t0 of the source at the own spatial site -> slab 0
+  if (live) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + s + S * (uint32_t)t0;
+#pragma unroll
+      for (int v = 0; v < 6; ++v) slab[0][r][v][threadIdx.x] = f[(size_t)v * vh];
+    }
+  }
+  __syncthreads();
+  Half<float> fifo[2][R];
+#pragma unroll
+  for (int k = 0; k < TB; ++k) {
+    const int t = t0 + k;
+    if (t >= LT) break;
+    const int cur = k & 1;
+    if (live) {
+      const Coord c{2 * xh + ((L.tcb + y + z + t) & 1), y, z, t};
+      const uint32_t tS = S * (uint32_t)t;
+      const uint32_t idx = s + tS;
+      Spinor<float> acc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) acc[r].s[sp][cc] = {0.f, 0.f};
+      {  // t hops: the forward neighbour also fills the slab of the next step
+        C<float> Wf[3][3], Wb[3][3];
+        load_pair<F32, G, false>(L.gauge + (size_t)3 * PV * vh, idx, vh, Wf, Wb);
+        const uint32_t nf = s + S * (uint32_t)(t + 1 == LT ? 0 : t + 1);
+        const uint32_t nb = s + S * (uint32_t)(t == 0 ? LT - 1 : t - 1);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride;
+          v4u buf[6];
+#pragma unroll
+          for (int v = 0; v < 6; ++v) buf[v] = f[(size_t)v * vh + nf];
+#pragma unroll
+          for (int v = 0; v < 6; ++v) slab[cur ^ 1][r][v][threadIdx.x] = buf[v];
+          const Spinor<float> pf = spinor_of(buf);
+          spin_recon_add<3, -IS>(acc[r], su3_mul(Wf, spin_project<3, -IS>(pf)));
+          Half<float> hb;
+          if (k < 2) hb = spin_project<3, IS>(load_spinor<F32>(f, nullptr, nb, vh));
+          else hb = fifo[k & 1][r];
+          spin_recon_add<3, IS>(acc[r], su3_mul(Wb, hb));
+          if (k + 2 < TB) fifo[k & 1][r] = spin_project<3, IS>(pf);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      lds_hop<G, IS, R, 0>(L, slab[cur], c, idx, s0, tS, r0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      lds_hop<G, IS, R, 1>(L, slab[cur], c, idx, s0, tS, r0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      lds_hop<G, IS, R, 2>(L, slab[cur], c, idx, s0, tS, r0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) store_spinor<F32, true>(L.res + (size_t)(r0 + r) * L.fstride, nullptr, idx, vh, acc[r]);
+    }
+    __syncthreads();  // slab[cur ^ 1] complete; slab[cur] free for the next step's writes
+  }
+  __syncthreads();  // the next tile's preamble overwrites slab 0
+}
+
+template <int G, int R, int IS>
+__global__ void __launch_bounds__(MRHS_LDS_THREADS) k_mrhs_lds(MrhsList list) {
+  __shared__ v4u slab[2][R][6][MRHS_LDS_THREADS];
+  const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
+  const uint32_t total = list.total_tiles;
+  const uint32_t x = blockIdx.x % NX;
+  const uint32_t per_x = gridDim.x / NX + (x < (gridDim.x % NX) ? 1u : 0u);
+  const uint32_t j = blockIdx.x / NX;
+  const uint32_t t_lo = (uint32_t)((uint64_t)total * x / NX), t_hi = (uint32_t)((uint64_t)total * (x + 1) / NX);
+  for (uint32_t t = t_lo + j; t < t_hi; t += per_x) {
+    int l = 0;
+    while (l + 1 < list.n && t >= list.lat[l + 1].tile0) ++l;
+    const MrhsLattice& L = list.lat[l];
+    // local tile = ((tc * ngroups + group) * nsb + sb) * rg + rb_in_group
+    const uint32_t local = t - L.tile0;
+    const uint32_t q = local / L.rg, rbi = local - q * L.rg;
+    const uint32_t q2 = q / L.nsb, sb = q - q2 * L.nsb;
+    const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
+    const int r0 = (int)(grp * L.rg + rbi) * R;
+    mrhs_march_lds<G, IS, R>(L, slab, sb * MRHS_LDS_THREADS, (int)tc * MRHS_T_BLOCK_LDS, r0);
+  }
+}
+
 template <int P, int G, int R, int TB>
 __global__ void __launch_bounds__(256) k_mrhs(MrhsList list) {
   const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
@@ -183,18 +345,28 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
       case 2: hipLaunchKernelGGL((k_mrhs<P, G, 2, 1>), dim3(grid), dim3(block), 0, s, list); break;
       default: hipLaunchKernelGGL((k_mrhs<P, G, 4, 1>), dim3(grid), dim3(block), 0, s, list); break;
     }
-  } else {
+  } else if (TB == MRHS_T_BLOCK) {
     if (R == 1) hipLaunchKernelGGL((k_mrhs<P, G, 1, MRHS_T_BLOCK>), dim3(grid), dim3(block), 0, s, list);
     else hipLaunchKernelGGL((k_mrhs<P, G, 2, MRHS_T_BLOCK>), dim3(grid), dim3(block), 0, s, list);
+  } else if constexpr (P == F32) {  // MRHS_T_BLOCK_LDS (validated by mrhs_shape_ok; one isign per list)
+    const bool plus = list.lat[0].isign > 0;
+    if (R == 1) {
+      if (plus) hipLaunchKernelGGL((k_mrhs_lds<G, 1, 1>), dim3(grid), dim3(MRHS_LDS_THREADS), 0, s, list);
+      else hipLaunchKernelGGL((k_mrhs_lds<G, 1, -1>), dim3(grid), dim3(MRHS_LDS_THREADS), 0, s, list);
+    } else {
+      if (plus) hipLaunchKernelGGL((k_mrhs_lds<G, 2, 1>), dim3(grid), dim3(MRHS_LDS_THREADS), 0, s, list);
+      else hipLaunchKernelGGL((k_mrhs_lds<G, 2, -1>), dim3(grid), dim3(MRHS_LDS_THREADS), 0, s, list);
+    }
   }
 }
 
 int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R, int block, int t_block,
                     MrhsList* list) {
   WD_REQUIRE(probs && nprob >= 1 && nprob <= MRHS_MAX_LATTICES, "1..%d problems per launch", MRHS_MAX_LATTICES);
-  WD_REQUIRE(R == 1 || R == 2 || R == 4, "rhs_block must be 1, 2 or 4");
-  WD_REQUIRE(t_block == 1 || t_block == MRHS_T_BLOCK, "t_block must be 1 or %d", MRHS_T_BLOCK);
-  WD_REQUIRE(t_block == 1 || R <= 2, "t-marching tiles are built for rhs_block 1 and 2");
+  WD_REQUIRE(mrhs_shape_ok(prec, R, block, t_block),
+             "unsupported batched launch shape (rhs_block %d, %d threads, t_block %d): t_block 1 takes rhs_block "
+             "1/2/4; t_block %d takes rhs_block 1/2; t_block %d takes F32, rhs_block 1/2 and %d threads",
+             R, block, t_block, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS);
   WD_REQUIRE(block >= 64 && block <= 256 && block % 64 == 0, "block_threads must be 64..256, multiple of 64");
   memset(list, 0, sizeof(*list));
   uint64_t tiles = 0;
@@ -224,12 +396,17 @@ int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R
     L.S = (uint32_t)(L.g.lxh * p.latt[1] * p.latt[2]);
     L.dlxh = make_fastdiv(L.g.lxh);
     L.dly = make_fastdiv(p.latt[1]);
+    WD_REQUIRE(t_block != MRHS_T_BLOCK_LDS || (p.nrhs % R == 0 && p.isign == probs[0].isign),
+               "problem %d: LDS-staged tiles need nrhs (%d) to be a multiple of rhs_block (%d) and one isign "
+               "for the whole launch", i, p.nrhs, R);
+    L.rg = L.nrb;
     if (t_block == 1) {
       L.nsb = (L.g.vh + block - 1) / block;
       L.ntc = 1;
     } else {
       L.nsb = (L.S + block - 1) / block;
       L.ntc = (p.latt[3] + t_block - 1) / t_block;
+      if (t_block == MRHS_T_BLOCK_LDS && L.nrb > MRHS_LDS_GROUP && L.nrb % MRHS_LDS_GROUP == 0) L.rg = MRHS_LDS_GROUP;
     }
     L.tile0 = (uint32_t)tiles;
     tiles += (uint64_t)L.ntc * L.nsb * L.nrb;

@@ -24,12 +24,13 @@ struct MrhsCfg {
   int R;
   int block;
   int grid;  // persistent grid blocks (0 for the one-tile-per-block launch)
-  int tb;    // t-slices per thread (1, or MRHS_T_BLOCK for t-marching tiles)
+  int tb;    // tile shape: 1, MRHS_T_BLOCK (t-marching) or MRHS_T_BLOCK_LDS (t-marching with an LDS slab)
 };
 
 // [format F32/F64][G==12][persistent]; defaults measured at 32 RHS on 24^3 x 48
 static MrhsCfg g_mrhs[2][2][2] = {
-    {{{2, 64, 0, MRHS_T_BLOCK}, {2, 64, 4096, MRHS_T_BLOCK}}, {{2, 64, 0, MRHS_T_BLOCK}, {2, 64, 4096, MRHS_T_BLOCK}}},
+    {{{2, MRHS_LDS_THREADS, 0, MRHS_T_BLOCK_LDS}, {1, MRHS_LDS_THREADS, 1024, MRHS_T_BLOCK_LDS}},
+     {{2, MRHS_LDS_THREADS, 0, MRHS_T_BLOCK_LDS}, {1, MRHS_LDS_THREADS, 1024, MRHS_T_BLOCK_LDS}}},
     {{{2, 128, 0, 1}, {2, 128, 2048, 1}}, {{2, 128, 0, 1}, {2, 128, 2048, 1}}}};
 
 static MrhsCfg& mrhs_entry(int prec, int G, int persistent) {
@@ -79,7 +80,12 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int
   const MrhsCfg& c = mrhs_entry(prec, G, grid_blocks > 0);
   int R = rhs_block ? rhs_block : c.R;
   int block = block_threads ? block_threads : c.block;
-  int tb = R > 2 ? 1 : c.tb;  // t-marching tiles exist for rhs_block 1 and 2
+  // the tabled tile shape, or the closest one this launch supports
+  int tb = c.tb;
+  if (tb == MRHS_T_BLOCK_LDS && (!mrhs_shape_ok(prec, R, block, tb) || !probs || nprob < 1 ||
+                                 !mrhs_lds_compatible(probs, nprob, R)))
+    tb = MRHS_T_BLOCK;
+  if (!mrhs_shape_ok(prec, R, block, tb)) tb = 1;
   return mrhs_run(probs, nprob, prec, G, R, block, grid_blocks, tb, (hipStream_t)stream);
 }
 
@@ -104,8 +110,9 @@ int dslash_set_mrhs_launch(int prec, int G, int persistent, int rhs_block, int b
   WD_REQUIRE(rhs_block == 1 || rhs_block == 2 || rhs_block == 4, "rhs_block must be 1, 2 or 4");
   WD_REQUIRE(block_threads >= 64 && block_threads <= 256 && block_threads % 64 == 0,
              "block_threads must be 64, 128, 192 or 256");
-  WD_REQUIRE(t_block == 1 || (t_block == MRHS_T_BLOCK && rhs_block <= 2),
-             "t_block must be 1, or %d with rhs_block 1 or 2", MRHS_T_BLOCK);
+  WD_REQUIRE(mrhs_shape_ok(prec, rhs_block, block_threads, t_block),
+             "t_block must be 1, %d (rhs_block 1 or 2) or %d (F32, rhs_block 1 or 2, %d threads)", MRHS_T_BLOCK,
+             MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS);
   WD_REQUIRE(persistent ? grid_blocks > 0 : grid_blocks == 0, "grid_blocks must be > 0 exactly when persistent");
   mrhs_entry(prec, G, persistent) = MrhsCfg{rhs_block, block_threads, grid_blocks, t_block};
   return 0;
@@ -163,6 +170,7 @@ int dslash_tune_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int 
   clear_error();
   WD_REQUIRE(prec == F32 || prec == F64, "multi-RHS Dslash is built for F32 and F64");
   WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
+  WD_REQUIRE(probs && nprob >= 1 && nprob <= MRHS_MAX_LATTICES, "1..%d problems per launch", MRHS_MAX_LATTICES);
   hipStream_t s = (hipStream_t)stream;
   MrhsCfg best{0, 0, 0, 1};
   float best_t = 1e30f;
@@ -170,11 +178,12 @@ int dslash_tune_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int 
   const int grids_np[] = {0};
   const int* grids = persistent ? grids_p : grids_np;
   const int ng = persistent ? 4 : 1;
-  for (int tb : {1, MRHS_T_BLOCK})
+  for (int tb : {1, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS})
     for (int R : {1, 2, 4}) {
-      if (tb > 1 && R > 2) continue;
+      if (tb == MRHS_T_BLOCK_LDS && !mrhs_lds_compatible(probs, nprob, R)) continue;
       for (int block : {64, 128, 256})
         for (int gi = 0; gi < ng; ++gi) {
+          if (!mrhs_shape_ok(prec, R, block, tb)) continue;
           float t = 0;
           int grid = grids[gi];
           int e = time_it(s, 10, [&]() { return mrhs_run(probs, nprob, prec, G, R, block, grid, tb, s); }, &t);
@@ -236,7 +245,7 @@ int dslash_tune_load(const char* path) {
       const int tb = n == 7 ? a[6] : 1;  // files written before t-marching tiles carry 6 fields
       if ((a[0] == F32 || a[0] == F64) && (a[1] == 18 || a[1] == 12) && (a[3] == 1 || a[3] == 2 || a[3] == 4) &&
           a[4] >= 64 && a[4] <= 256 && a[4] % 64 == 0 && a[5] >= 0 &&
-          (tb == 1 || (tb == MRHS_T_BLOCK && a[3] <= 2)))
+          mrhs_shape_ok(a[0], a[3], a[4], tb))
         mrhs_entry(a[0], a[1], a[2]) = MrhsCfg{a[3], a[4], a[2] ? (a[5] > 0 ? a[5] : 2048) : 0, tb};
       else
         ++bad;

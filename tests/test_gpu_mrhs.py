@@ -83,6 +83,43 @@ def test_32_rhs_24c48(oracle, prec, G):
     check_item(oracle, item, u, host, 1e-5 if prec == F32 else 1e-13)
 
 
+@pytest.mark.parametrize("rhs_block,nrhs", [(1, 7), (2, 8)])
+@pytest.mark.parametrize("latt", [(8, 8, 8, 16), (8, 4, 6, 10), (12, 6, 4, 8)])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_lds_staged_tiles(oracle, launch_table, rhs_block, nrhs, latt, isign):
+    """t_block 8: 128-thread blocks walking 8 slices with an LDS slab (T = 10 leaves
+    a 2-slice chunk; 12x6x4 has 144 spatial sites, so one block is partial)."""
+    item, u, host = make_item(latt, nrhs, isign=isign, target_cb=0)
+    set_launch(F32, 18, False, rhs_block, 128, 0, 8)
+    apply_batch([item], rhs_block=rhs_block, block_threads=128)
+    torch.cuda.synchronize()
+    op = item.op
+    for k in range(nrhs):
+        single = op.new_spinor(False)
+        op.apply_cb(single, item.psi.field(k), isign, 0)
+        a = op.unpack_cb(single, 0, dtype=np.float32)
+        b = op.unpack_cb(item.res.field(k), 0, dtype=np.float32)
+        assert R.rel_err(b, a) < 1e-6, (k, R.rel_err(b, a))
+    check_item(oracle, item, u, host, 1e-5, ks=[0, nrhs - 1])
+
+
+def test_lds_tiles_persistent_and_fallback(oracle, launch_table):
+    """A persistent LDS-staged launch over two lattices with one isign; with
+    mixed isign the tabled LDS shape falls back to the register march."""
+    items = [make_item((8, 8, 8, 8), 4, F32, 18, 1, 1, 6), make_item((16, 8, 8, 16), 6, F32, 18, 1, 1, 7)]
+    set_launch(F32, 18, True, 2, 128, 64, 8)
+    apply_batch([it[0] for it in items], grid_blocks=64)
+    torch.cuda.synchronize()
+    for item, u, host in items:
+        check_item(oracle, item, u, host, 1e-5)
+    mixed = uneven_items()
+    set_launch(F32, 18, False, 2, 128, 0, 8)
+    apply_batch([it[0] for it in mixed])
+    torch.cuda.synchronize()
+    for item, u, host in mixed:
+        check_item(oracle, item, u, host, 1e-5, ks=[0, 3])
+
+
 def uneven_items():
     out = []
     for latt, isign, tcb, seed in (((8, 8, 8, 8), 1, 0, 3), ((16, 16, 16, 32), -1, 1, 4), ((24, 24, 24, 48), 1, 1, 5)):

@@ -181,14 +181,20 @@ def test_mrhs_launch_table_and_tune_file(tmp_path):
     v = [C.c_int() for _ in range(4)]
     assert lib.dslash_get_mrhs_launch(_lib.F32, 18, 0, *[C.byref(x) for x in v]) == 0
     saved = [x.value for x in v]
+    lib.dslash_get_mrhs_launch(_lib.F32, 12, 1, *[C.byref(x) for x in v])
+    saved12 = [x.value for x in v]
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 4, 128, 0, 4) == _lib.ERR_INVALID  # no R=4 marching
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 128, 512, 1) == _lib.ERR_INVALID  # grid only if persistent
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 192, 0, 3) == _lib.ERR_INVALID
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 64, 0, 8) == _lib.ERR_INVALID  # LDS tiles: 128 threads
+    assert lib.dslash_set_mrhs_launch(_lib.F64, 18, 0, 2, 128, 0, 8) == _lib.ERR_INVALID  # LDS tiles: F32 only
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 12, 1, 2, 128, 1024, 8) == 0
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 192, 0, 4) == 0
     path = str(tmp_path / "tune.txt")
     assert lib.dslash_tune_save(path.encode()) == 0
     assert "mrhs 0 18 0 1 192 0 4\n" in open(path).read()
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, *saved) == 0
+    assert "mrhs 0 12 1 2 128 1024 8\n" in open(path).read()
     assert lib.dslash_tune_load(path.encode()) == 0
     lib.dslash_get_mrhs_launch(_lib.F32, 18, 0, *[C.byref(x) for x in v])
     assert [x.value for x in v] == [1, 192, 0, 4]
@@ -198,6 +204,7 @@ def test_mrhs_launch_table_and_tune_file(tmp_path):
     lib.dslash_get_mrhs_launch(_lib.F32, 18, 0, *[C.byref(x) for x in v])
     assert [x.value for x in v] == [2, 128, 0, 1]
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, *saved) == 0
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 12, 1, *saved12) == 0
 
 
 def test_face_layout():

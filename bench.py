@@ -59,7 +59,7 @@ def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=Tr
     psi = op.gen_spinor(seed=1, field=0)
     res = op.new_spinor(zero=True)
     torch.cuda.synchronize()
-    shape = tune_apply(op, res, psi)  # picks threads/block and XCD order for this (nd, format, links)
+    shape = tune_apply(op, res, psi)  # picks threads/block, XCD order and marching length
     op.time_apply(res, psi, warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -74,7 +74,7 @@ def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=Tr
         "gbs": byts / (ms * 1e-3) / 1e9,
         "bytes_per_call": byts,
         "wall_s": wall,
-        "launch": {"block_threads": shape["block_threads"], "xcd_remap": shape["xcd_remap"]},
+        "launch": {"block_threads": shape["block_threads"], "xcd_remap": shape["xcd_remap"], "march": shape["march"]},
     }
     if verbose:
         log(f"[gpu] {prec_name} nd={nd} G={G}: {ms * 1e3:.1f} us/call, {out['gflops']:.0f} GFLOP/s, "

@@ -125,6 +125,20 @@ int dslash_apply_full(void* d_res, const void* d_psi, const void* d_gauge,
  * dslash_tune_apply (below) measures and sets the best entry. */
 int dslash_set_launch(int nd, int prec, int gauge_reals, int block_threads, int xcd_remap);
 int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int* xcd_remap);
+/* Marching length of res = D psi for (nd, prec, gauge_reals): 1 = one target
+ * site per thread; 4 or 8 = a thread walks that many consecutive slices of the
+ * slowest coordinate of the site order (t in 4D, z in 3D) and keeps the
+ * projected source spinor of its forward hop for the backward hop two slices
+ * later.  Any lattice extent is allowed (the last chunk is shorter).  A launch
+ * that would have fewer than 131072 threads at this length (small lattices,
+ * short t or z) uses the next shorter one, down to 1.  The x + a D psi forms
+ * always run one site per thread.  dslash_set_launch leaves it unchanged;
+ * dslash_tune_apply tunes it with the block shape. */
+int dslash_set_launch_march(int nd, int prec, int gauge_reals, int march);
+int dslash_get_launch_march(int nd, int prec, int gauge_reals, int* march);
+/* The marching length res = D psi actually runs with on this lattice (the set
+ * one, or shorter where it would leave fewer than 131072 threads). */
+int dslash_launch_march_for(const int latt[4], int nd, int prec, int gauge_reals, int* march);
 
 /* ---- many right-hand sides, many lattices ----------------------------- */
 /* One lattice of a batched launch: nrhs single-checkerboard fields on one
@@ -164,7 +178,8 @@ int dslash_set_mrhs_launch(int prec, int gauge_reals, int persistent, int rhs_bl
                            int grid_blocks, int t_block);
 
 /* ---- autotuning -------------------------------------------------------- */
-/* Time every launch shape of the single-RHS kernel (block size x XCD remap) on
+/* Time every launch shape of the single-RHS kernel (block size x XCD remap x
+ * marching length) on
  * the given full-lattice fields (dslash_apply_full, alternating isign and the
  * source checkerboard; both halves of res are overwritten), keep the fastest
  * for (nd, prec, gauge_reals) and report it. */

@@ -64,6 +64,9 @@ PROTOTYPES = {
     "dslash_apply_xpay": (_i, [_vp, _vp, _vp, _vp, C.c_double, _ip, _i, _i, _i, _i, _vp]),
     "dslash_set_launch": (_i, [_i, _i, _i, _i, _i]),
     "dslash_get_launch": (_i, [_i, _i, _i, _ip, _ip]),
+    "dslash_set_launch_march": (_i, [_i, _i, _i, _i]),
+    "dslash_get_launch_march": (_i, [_i, _i, _i, _ip]),
+    "dslash_launch_march_for": (_i, [_ip, _i, _i, _i, _ip]),
     "dslash_apply_mrhs": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp]),
     "dslash_tune_apply": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _vp, _ip, _ip, _fp]),
     "dslash_tune_mrhs": (_i, [_vp, _i, _i, _i, _i, _vp, _ip, _ip, _ip, _fp]),

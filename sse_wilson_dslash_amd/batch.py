@@ -103,10 +103,11 @@ def set_launch(prec: int, G: int, persistent: bool, rhs_block: int, block_thread
 
 def tune_apply(op: WilsonDslash, res: torch.Tensor, psi: torch.Tensor) -> dict:
     """Tune the single-RHS launch shape of op's (nd, format, link reals)."""
-    B, Rm, ms = C.c_int(), C.c_int(), C.c_float()
+    B, Rm, M, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
     check(op.lib.dslash_tune_apply(ptr(res), ptr(psi), ptr(op.gauge), op._L, op.nd, op.prec, op.G,
                                    current_stream(), C.byref(B), C.byref(Rm), C.byref(ms)))
-    return {"block_threads": B.value, "xcd_remap": Rm.value, "ms": ms.value}
+    check(op.lib.dslash_get_launch_march(op.nd, op.prec, op.G, C.byref(M)))
+    return {"block_threads": B.value, "xcd_remap": Rm.value, "march": M.value, "ms": ms.value}
 
 
 def flops(items) -> int:

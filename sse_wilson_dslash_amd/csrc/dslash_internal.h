@@ -10,7 +10,9 @@ namespace wd {
 struct LaunchCfg {
   int block;  // threads per block
   int remap;  // XCD-contiguous block order
+  int march;  // slices of the slowest coordinate per thread in res = D psi (1, 4 or 8)
 };
+inline bool march_ok(int m) { return m == 1 || m == 4 || m == 8; }
 LaunchCfg launch_cfg(int nd, int prec, int G);
 void set_launch_cfg(int nd, int prec, int G, LaunchCfg c);
 

@@ -53,13 +53,23 @@ void clear_error() {
 // ---------------------------------------------------------------------------
 // launch configuration
 // ---------------------------------------------------------------------------
-// Threads per block and XCD remapping per (4D/3D, format, link reals); set by
-// dslash_set_launch or by the autotuner.  Defaults are the shapes measured
-// fastest at 32^3 x 64 on MI355X (XCD-contiguous order is worth ~25%).
+// Threads per block, XCD remapping and marching length per (4D/3D, format,
+// link reals); set by dslash_set_launch / dslash_set_launch_march or by the
+// autotuner.  Defaults are the shapes measured fastest at 32^3 x 64 on MI355X
+// (XCD-contiguous order is worth ~25%; marching 6-16% for FP64, sloppy FP64 and
+// the 3D operator, ~1% for 4D FP32, nothing for 16-bit spinors).
 // Index: [nd == 4][F32, F64, F16, BF16, F64S][G == 12].
 static LaunchCfg g_cfg[2][NUM_PREC][2] = {
-    {{{128, 1}, {128, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{128, 1}, {128, 1}}},
-    {{{256, 1}, {256, 1}}, {{128, 1}, {128, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}, {{256, 1}, {256, 1}}}};
+    {{{256, 1, 8}, {256, 1, 8}},
+     {{64, 1, 4}, {256, 1, 8}},
+     {{64, 1, 1}, {64, 1, 8}},
+     {{64, 1, 1}, {64, 1, 1}},
+     {{256, 1, 8}, {256, 1, 8}}},
+    {{{256, 1, 8}, {256, 1, 8}},
+     {{256, 1, 8}, {64, 1, 8}},
+     {{256, 1, 1}, {256, 1, 1}},
+     {{256, 1, 1}, {256, 1, 1}},
+     {{256, 1, 8}, {256, 1, 8}}}};
 
 static int prec_slot(int prec) { return prec >= 0 && prec < NUM_PREC ? prec : 0; }
 LaunchCfg launch_cfg(int nd, int prec, int G) { return g_cfg[nd == 4][prec_slot(prec)][G == 12]; }
@@ -100,6 +110,9 @@ struct ApplyArgs {
   int tcb;
   int remap;
   int block;
+  int march;        // slices per thread of k_dslash_march (1 = k_dslash)
+  uint32_t slice;   // sites per slice of the slowest coordinate (t in 4D, z in 3D)
+  FastDiv dslice;
 };
 
 template <int ND, int P, int G, int IS, int MU>
@@ -194,9 +207,91 @@ k_dslash(ApplyArgs a) {
   }
 }
 
+// Fewest threads a marching launch may have: 512 blocks of 256, two per CU.
+constexpr uint64_t MARCH_MIN_THREADS = 131072;
+
+// Marching form of res = D psi.  The site order runs slowest in t (4D) or z
+// (3D), so the neighbours of site s + S*m in that direction are s + S*(m +- 1)
+// with S = Vh / L: a thread owns one site s of a slice and walks TB consecutive
+// slices m.  The source spinor it gathers for the forward hop of slice m is the
+// backward neighbour of slice m + 2, so its projection (1 + isign gamma) psi
+// stays in registers for two slices instead of being gathered again from S
+// sites away, where at large S the second read misses L2.
+template <int ND, int P, int G, int IS, int TB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Fmt<P>::half ? 2 : 1, 8)))
+k_dslash_march(ApplyArgs a) {
+  typedef typename Fmt<P>::compute T;
+  constexpr int MD = ND - 1;  // marching direction: t (4D), z (3D)
+  constexpr int PV = PairVec<P, G>::value;
+  const uint32_t b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t p = b * blockDim.x + threadIdx.x;
+  const uint32_t S = a.slice;
+  const uint32_t chunk = fdiv(p, a.dslice);
+  const uint32_t s = p - chunk * S;
+  const int L = a.g.L[MD];
+  const int m0 = (int)chunk * TB;
+  if (m0 >= L) return;
+  const Coord c0 = site_coords<ND>(a.g, s, 0);  // s < S: coordinate MD is 0
+  const uint32_t vh = a.g.vh;
+  Half<T> keep[2];
+#pragma unroll
+  for (int k = 0; k < TB; ++k) {
+    const int m = m0 + k;
+    if (m >= L) break;
+    Coord c = c0;
+    if constexpr (ND == 4)
+      c.t = m;
+    else
+      c.z = m;
+    c.x = (c0.x & ~1) | ((a.tcb + c.y + c.z + (ND == 4 ? c.t : 0)) & 1);
+    const uint32_t idx = s + S * (uint32_t)m;
+    Spinor<T> acc;
+#pragma unroll
+    for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) acc.s[sp][cc] = {(T)0, (T)0};
+    hop_dir<ND, P, G, IS, 0>(a, c, idx, acc);
+    if constexpr (Fmt<P>::half) __builtin_amdgcn_sched_barrier(0);
+    hop_dir<ND, P, G, IS, 1>(a, c, idx, acc);
+    if constexpr (ND == 4) {
+      if constexpr (Fmt<P>::half) __builtin_amdgcn_sched_barrier(0);
+      hop_dir<ND, P, G, IS, 2>(a, c, idx, acc);
+    }
+    if constexpr (Fmt<P>::half) __builtin_amdgcn_sched_barrier(0);
+    {
+      C<T> Wf[3][3], Wb[3][3];
+      load_pair<P, G>(a.gauge + (size_t)MD * PV * vh, idx, vh, Wf, Wb);
+      const int mf = m + 1 == L ? 0 : m + 1;
+      const Spinor<T> pf = load_spinor<P>(a.psi, a.psi_norm, s + S * (uint32_t)mf, vh);
+      spin_recon_add<MD, -IS>(acc, su3_mul(Wf, spin_project<MD, -IS>(pf)));
+      Half<T> hb;
+      if (k < 2) {
+        const int mb = m == 0 ? L - 1 : m - 1;
+        hb = spin_project<MD, IS>(load_spinor<P>(a.psi, a.psi_norm, s + S * (uint32_t)mb, vh));
+      } else {
+        hb = keep[k & 1];
+      }
+      spin_recon_add<MD, IS>(acc, su3_mul(Wb, hb));
+      if (k + 2 < TB) keep[k & 1] = spin_project<MD, IS>(pf);
+    }
+    store_spinor<P, true>(a.res, a.res_norm, idx, vh, acc);
+    // keeps the next slice's loads from being hoisted into this one's registers
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int ND, int P, int G, int IS>
 static void launch_dslash_t(const ApplyArgs& a, hipStream_t s) {
   uint32_t blocks = (a.g.vh + a.block - 1) / a.block;
+  if (!a.xp && a.march > 1) {
+    const uint32_t chunks = (uint32_t)(a.g.L[ND - 1] + a.march - 1) / (uint32_t)a.march;
+    const uint32_t mblocks = (a.slice * chunks + a.block - 1) / a.block;
+    if (a.march == 4)
+      hipLaunchKernelGGL((k_dslash_march<ND, P, G, IS, 4>), dim3(mblocks), dim3(a.block), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_dslash_march<ND, P, G, IS, 8>), dim3(mblocks), dim3(a.block), 0, s, a);
+    return;
+  }
   if (a.xp) {
     if constexpr (ND == 4) {
       if (a.norm_partial)
@@ -236,6 +331,18 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
   }
 }
 
+// Marching divides the thread count by the march length; on lattices with few
+// sites per slice or few slices that would leave CUs idle, so a shorter march
+// (down to one site per thread) is used when the launch would fall below
+// MARCH_MIN_THREADS threads.
+static int effective_march(const int latt[4], int nd, int march) {
+  const uint64_t L = (uint64_t)latt[nd - 1];
+  const uint64_t slice = (uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2 / L;
+  int m = march_ok(march) ? march : 1;
+  while (m > 1 && slice * ((L + m - 1) / m) < MARCH_MIN_THREADS) m = m == 8 ? 4 : 1;
+  return m;
+}
+
 int apply_blocks(const int latt[4], int nd, int prec, int G) {
   const uint32_t vh = (uint32_t)((uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2);
   const int blk = launch_cfg(nd, prec, G).block;
@@ -270,6 +377,9 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
   const LaunchCfg lc = launch_cfg(nd, prec, G);
   a.remap = lc.remap;
   a.block = lc.block;
+  a.slice = a.g.vh / (uint32_t)latt[nd - 1];
+  a.dslice = make_fastdiv(a.slice);
+  a.march = effective_march(latt, nd, lc.march);
   if (nd == 4)
     launch_dslash_nd<4>(a, prec, G, isign, s);
   else
@@ -478,17 +588,56 @@ int dslash_set_launch(int nd, int prec, int gauge_reals, int block_threads, int 
   clear_error();
   WD_REQUIRE(block_threads >= 64 && block_threads <= 256 && block_threads % 64 == 0,
              "block_threads must be 64, 128, 192 or 256 (got %d)", block_threads);
-  LaunchCfg c{block_threads, xcd_remap ? 1 : 0};
   if (nd == 0) {
     for (int n : {3, 4})
       for (int p = 0; p < NUM_PREC; ++p)
-        for (int G : {18, 12}) set_launch_cfg(n, p, G, c);
+        for (int G : {18, 12}) {
+          LaunchCfg c = launch_cfg(n, p, G);
+          c.block = block_threads;
+          c.remap = xcd_remap ? 1 : 0;
+          set_launch_cfg(n, p, G, c);
+        }
     return 0;
   }
   WD_REQUIRE(nd == 3 || nd == 4, "nd must be 0 (all), 3 or 4");
   if (int e = check_prec(prec)) return e;
   WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
+  LaunchCfg c = launch_cfg(nd, prec, gauge_reals);
+  c.block = block_threads;
+  c.remap = xcd_remap ? 1 : 0;
   set_launch_cfg(nd, prec, gauge_reals, c);
+  return 0;
+}
+
+int dslash_set_launch_march(int nd, int prec, int gauge_reals, int march) {
+  clear_error();
+  WD_REQUIRE(march_ok(march), "march must be 1, 4 or 8 (got %d)", march);
+  WD_REQUIRE(nd == 3 || nd == 4, "nd must be 3 or 4");
+  if (int e = check_prec(prec)) return e;
+  WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
+  LaunchCfg c = launch_cfg(nd, prec, gauge_reals);
+  c.march = march;
+  set_launch_cfg(nd, prec, gauge_reals, c);
+  return 0;
+}
+
+int dslash_launch_march_for(const int latt[4], int nd, int prec, int gauge_reals, int* march) {
+  clear_error();
+  if (int e = check_lattice(latt, nd)) return e;
+  if (int e = check_prec(prec)) return e;
+  WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
+  WD_REQUIRE(march, "march is NULL");
+  *march = effective_march(latt, nd, launch_cfg(nd, prec, gauge_reals).march);
+  return 0;
+}
+
+int dslash_get_launch_march(int nd, int prec, int gauge_reals, int* march) {
+  clear_error();
+  WD_REQUIRE(nd == 3 || nd == 4, "nd must be 3 or 4");
+  if (int e = check_prec(prec)) return e;
+  WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
+  WD_REQUIRE(march, "march is NULL");
+  *march = launch_cfg(nd, prec, gauge_reals).march;
   return 0;
 }
 

@@ -131,33 +131,38 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   // Candidates are timed in three interleaved rounds and scored by their best
   // round, so a transient slowdown of the device does not pick a shape.
   const int blocks[4] = {64, 128, 192, 256};
-  float tmin[4][2];
-  for (auto& row : tmin) row[0] = row[1] = 1e30f;
+  const int marches[3] = {1, 4, 8};
+  float tmin[4][2][3];
+  for (auto& a : tmin)
+    for (auto& b : a)
+      for (float& t : b) t = 1e30f;
   for (int round = 0; round < 3; ++round)
     for (int bi = 0; bi < 4; ++bi)
-      for (int remap : {0, 1}) {
-        set_launch_cfg(nd, prec, G, LaunchCfg{blocks[bi], remap});
-        float t = 0;
-        int k = 0;
-        // the same alternation of isign and source checkerboard as a solver sees
-        int e = time_it(s, reps, [&]() {
-          ++k;
-          return dslash_apply_full(d_res, d_psi, d_gauge, latt, nd, prec, G, (k & 2) ? -1 : 1, k & 1, stream);
-        }, &t);
-        if (e) {
-          set_launch_cfg(nd, prec, G, saved);
-          return e;
+      for (int remap : {0, 1})
+        for (int mi = 0; mi < 3; ++mi) {
+          set_launch_cfg(nd, prec, G, LaunchCfg{blocks[bi], remap, marches[mi]});
+          float t = 0;
+          int k = 0;
+          // the same alternation of isign and source checkerboard as a solver sees
+          int e = time_it(s, reps, [&]() {
+            ++k;
+            return dslash_apply_full(d_res, d_psi, d_gauge, latt, nd, prec, G, (k & 2) ? -1 : 1, k & 1, stream);
+          }, &t);
+          if (e) {
+            set_launch_cfg(nd, prec, G, saved);
+            return e;
+          }
+          if (t < tmin[bi][remap][mi]) tmin[bi][remap][mi] = t;
         }
-        if (t < tmin[bi][remap]) tmin[bi][remap] = t;
-      }
   LaunchCfg best = saved;
   float best_t = 1e30f;
   for (int bi = 0; bi < 4; ++bi)
     for (int remap : {0, 1})
-      if (tmin[bi][remap] < best_t) {
-        best_t = tmin[bi][remap];
-        best = LaunchCfg{blocks[bi], remap};
-      }
+      for (int mi = 0; mi < 3; ++mi)
+        if (tmin[bi][remap][mi] < best_t) {
+          best_t = tmin[bi][remap][mi];
+          best = LaunchCfg{blocks[bi], remap, marches[mi]};
+        }
   set_launch_cfg(nd, prec, G, best);
   if (block_threads) *block_threads = best.block;
   if (xcd_remap) *xcd_remap = best.remap;
@@ -212,7 +217,7 @@ int dslash_tune_save(const char* path) {
     for (int p = 0; p < NUM_PREC; ++p)
       for (int G : {18, 12}) {
         LaunchCfg c = launch_cfg(nd, p, G);
-        fprintf(f, "apply %d %d %d %d %d\n", nd, p, G, c.block, c.remap);
+        fprintf(f, "apply %d %d %d %d %d %d\n", nd, p, G, c.block, c.remap, c.march);
       }
   for (int p : {F32, F64})
     for (int G : {18, 12})
@@ -234,10 +239,11 @@ int dslash_tune_load(const char* path) {
   while (fgets(line, sizeof(line), f)) {
     int a[7];
     if (line[0] == '#') continue;
-    if (sscanf(line, "apply %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4]) == 5) {
+    if (int n = sscanf(line, "apply %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]); n >= 5) {
+      const int march = n == 6 ? a[5] : 1;  // files written before marching carry 5 fields
       if ((a[0] == 3 || a[0] == 4) && a[1] >= 0 && a[1] < NUM_PREC && (a[2] == 18 || a[2] == 12) && a[3] >= 64 &&
-          a[3] <= 256 && a[3] % 64 == 0)
-        set_launch_cfg(a[0], a[1], a[2], LaunchCfg{a[3], a[4] ? 1 : 0});
+          a[3] <= 256 && a[3] % 64 == 0 && march_ok(march))
+        set_launch_cfg(a[0], a[1], a[2], LaunchCfg{a[3], a[4] ? 1 : 0, march});
       else
         ++bad;
     } else if (int n = sscanf(line, "mrhs %d %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &a[6]);

@@ -196,6 +196,7 @@ def test_autotuner(oracle):
     res = op.new_spinor(zero=True)
     t = tune_apply(op, res, psi)
     assert t["block_threads"] in (64, 128, 192, 256) and t["xcd_remap"] in (0, 1) and t["ms"] > 0
+    assert t["march"] in (1, 4, 8)
     # the tuned shape still computes the same operator
     op.apply(res, psi, 1, 0)
     got = op.unpack_spinor(res, dtype=np.float32)

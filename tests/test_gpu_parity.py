@@ -192,6 +192,119 @@ def test_launch_shapes_agree(oracle, block, remap):
     check(got, want, latt, 0, F32)
 
 
+MARCH_TOL = {F64: 1e-13, F32: 1e-5, F64S: 1e-6, F16: 2e-3, BF16: 1.6e-2}
+# Largest difference between two marching lengths, relative to max |res|: FP32
+# arithmetic may contract differently in the two kernels, which for 16-bit
+# results can move a stored value by one unit of its format (2^-10, 2^-7 of the
+# site's norm).
+MARCH_SAME = {F64: 1e-13, F32: 1e-6, F64S: 1e-6, F16: 2.0 ** -10, BF16: 2.0 ** -7}
+
+
+def _march_for(latt, nd, prec, G):
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    m = C.c_int()
+    _lib.check(_lib.core().dslash_launch_march_for(_lib.latt_arg(latt), nd, prec, G, C.byref(m)))
+    return m.value
+
+
+class _March:
+    """Sets the marching length of (nd, prec, G) and restores it on exit."""
+
+    def __init__(self, nd, prec, G):
+        import ctypes as C
+        from sse_wilson_dslash_amd import _lib
+        self.lib, self.check, self.key = _lib.core(), _lib.check, (nd, prec, G)
+        m = C.c_int()
+        self.check(self.lib.dslash_get_launch_march(nd, prec, G, C.byref(m)))
+        self.saved = m.value
+
+    def set(self, march):
+        self.check(self.lib.dslash_set_launch_march(*self.key, march))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.set(self.saved)
+
+
+@pytest.mark.parametrize("nd,latt", [(4, (4, 6, 4, 10)), (4, (2, 2, 2, 2)), (3, (4, 2, 6, 3)), (3, (2, 4, 10, 2))])
+@pytest.mark.parametrize("prec", [F64, F32, F16])
+def test_marching_falls_back_on_small_lattices(oracle, nd, latt, prec):
+    """A marching length set for the format is not used where the launch would
+    have too few threads to fill the device: these lattices run one site per
+    thread whatever is set, and agree with the oracle."""
+    print(f"[march small] nd={nd} latt={latt} prec={prec}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=3)
+    psi = gen_spinor_host(latt, nd, seed=3, field=2)
+    if prec != F64:
+        u, psi = u.astype(np.float32), psi.astype(np.float32)
+    vh = volume(latt) // 2
+    with _March(nd, prec, 12) as m:
+        for march in (4, 8):
+            m.set(march)
+            assert _march_for(latt, nd, prec, 12) == 1
+            for isign, cb in ((1, 0), (-1, 1)):
+                got = run_gpu(u, psi, latt, nd, prec, 12, isign, cb)
+                want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
+                t = slice((1 - cb) * vh, (2 - cb) * vh)
+                assert R.rel_err(got.reshape(-1, 24)[t], want.reshape(-1, 24)[t]) < MARCH_TOL[prec]
+
+
+# Lattices on which marching runs: >= 131072 threads at march 8.  t = 66 and
+# z = 34 leave a 2-slice last chunk at march 4 and 8; t = 2 and z = 2 are shorter
+# than the march; t = 65 is an odd time extent of the 3D operator.
+MARCH_CASES = [(4, (32, 32, 32, 66), F64, 18), (4, (32, 32, 32, 66), F32, 12), (4, (32, 32, 32, 66), BF16, 12),
+               (4, (64, 64, 64, 2), F64S, 18), (4, (64, 64, 64, 2), F16, 12),
+               (3, (32, 32, 34, 65), F64, 12), (3, (32, 32, 34, 65), F32, 18), (3, (32, 32, 34, 65), F16, 18),
+               (3, (64, 64, 2, 64), BF16, 18), (3, (64, 64, 2, 64), F64S, 12)]
+
+
+@pytest.mark.parametrize("nd,latt,prec,G", MARCH_CASES)
+def test_marching_lengths(oracle, nd, latt, prec, G):
+    """res = D psi with a thread walking 1, 4 or 8 slices of the slowest
+    coordinate (t in 4D, z in 3D): every march agrees with the oracle and with
+    the one-site-per-thread result, and leaves the source checkerboard of res
+    untouched."""
+    print(f"[march] nd={nd} latt={latt} prec={prec} G={G}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=3)
+    psi = gen_spinor_host(latt, nd, seed=3, field=2)
+    if prec == F32:
+        u, psi = u.astype(np.float32), psi.astype(np.float32)
+    dt = np.float32 if prec in (F32, F16, BF16) else np.float64
+    vh = volume(latt) // 2
+    op = WilsonDslash(latt, nd=nd, prec=prec, gauge_reals=G)
+    op.load_gauge(u)
+    d_psi = op.pack_spinor(psi)
+    with _March(nd, prec, G) as m:
+        for isign, cb in ((1, 0), (-1, 1)):
+            want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
+            t = slice((1 - cb) * vh, (2 - cb) * vh)
+            w = want.reshape(-1, 24)[t]
+            first = None
+            for march in (1, 4, 8):
+                m.set(march)
+                assert _march_for(latt, nd, prec, G) == march
+                d_res = op.pack_spinor(np.full_like(psi, 3.25))
+                op.apply(d_res, d_psi, isign, cb)
+                got = op.unpack_spinor(d_res, dtype=dt).reshape(-1, 24)
+                g = got[t]
+                e2, einf = R.rel_err(g, w), R.max_rel_err(g, w)
+                print(f"  isign={isign:+d} cb={cb} march={march}: rel L2 {e2:.2e} rel max {einf:.2e}", flush=True)
+                assert e2 < MARCH_TOL[prec], (march, isign, cb, e2)
+                if prec in (F64, F32, F64S):
+                    assert einf < MARCH_TOL[prec], (march, isign, cb, einf)
+                assert np.all(got[cb * vh:(cb + 1) * vh] == 3.25), "source cb of res modified"
+                if first is None:
+                    first = g
+                else:
+                    assert np.max(np.abs(g - first)) <= MARCH_SAME[prec] * np.max(np.abs(first)), \
+                        (march, isign, cb)
+    del op, d_psi
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("prec", [F64, F32])
 def test_xpay_epilogue(oracle, prec):
     """dslash_apply_xpay: res = x + a D psi in one pass (and in place, x aliasing res)."""

@@ -157,21 +157,90 @@ def test_sizes_and_validation():
     assert lib.dslash_set_launch(4, _lib.F32, 18, *saved) == 0
 
 
+def test_march_setting():
+    lib = _lib.core()
+    m = C.c_int()
+    assert lib.dslash_get_launch_march(3, _lib.F32, 18, C.byref(m)) == 0
+    saved = m.value
+    assert saved in (1, 4, 8)
+    for bad in (0, 2, 3, 16, -1):
+        assert lib.dslash_set_launch_march(3, _lib.F32, 18, bad) == _lib.ERR_INVALID
+    assert b"march" in lib.dslash_last_error_message()
+    assert lib.dslash_set_launch_march(5, _lib.F32, 18, 4) == _lib.ERR_INVALID
+    assert lib.dslash_set_launch_march(3, _lib.F32, 18, 4) == 0
+    b, r = C.c_int(), C.c_int()
+    lib.dslash_get_launch(3, _lib.F32, 18, C.byref(b), C.byref(r))
+    assert lib.dslash_set_launch(3, _lib.F32, 18, 64, 0) == 0  # leaves the march alone
+    lib.dslash_get_launch_march(3, _lib.F32, 18, C.byref(m))
+    assert m.value == 4
+    assert lib.dslash_set_launch(3, _lib.F32, 18, b.value, r.value) == 0
+    assert lib.dslash_set_launch_march(3, _lib.F32, 18, saved) == 0
+
+
+def test_march_falls_back_below_the_thread_floor():
+    lib = _lib.core()
+    m = C.c_int()
+    lib.dslash_get_launch_march(4, _lib.F64, 18, C.byref(m))
+    saved4 = m.value
+    lib.dslash_get_launch_march(3, _lib.F64, 18, C.byref(m))
+    saved3 = m.value
+
+    def eff(latt, nd):
+        assert lib.dslash_launch_march_for(_lib.latt_arg(latt), nd, _lib.F64, 18, C.byref(m)) == 0
+        return m.value
+    try:
+        assert lib.dslash_set_launch_march(4, _lib.F64, 18, 8) == 0
+        assert lib.dslash_set_launch_march(3, _lib.F64, 18, 8) == 0
+        # 4D: 16384 sites per t-slice; threads = 16384 * ceil(Lt / march) >= 131072
+        assert eff((32, 32, 32, 64), 4) == 8   # 8 chunks
+        assert eff((32, 32, 32, 66), 4) == 8   # 9 chunks, the last of 2 slices
+        assert eff((32, 32, 32, 32), 4) == 4   # 8: 65536 threads; 4: 131072
+        assert eff((32, 32, 32, 8), 4) == 1
+        assert eff((4, 4, 4, 4), 4) == 1
+        assert eff((64, 64, 64, 2), 4) == 8    # 131072 sites per slice
+        # 3D: z is the slowest coordinate; 32768 sites per z-slice at 32^2 x 64
+        assert eff((32, 32, 32, 64), 3) == 8
+        assert eff((32, 32, 16, 64), 3) == 4
+        assert eff((16, 16, 16, 64), 3) == 1
+        assert lib.dslash_set_launch_march(4, _lib.F64, 18, 1) == 0
+        assert eff((32, 32, 32, 64), 4) == 1
+        assert lib.dslash_launch_march_for(_lib.latt_arg((3, 4, 4, 4)), 4, _lib.F64, 18, C.byref(m)) == \
+            _lib.ERR_INVALID
+    finally:
+        lib.dslash_set_launch_march(4, _lib.F64, 18, saved4)
+        lib.dslash_set_launch_march(3, _lib.F64, 18, saved3)
+
+
 def test_tune_file_round_trip(tmp_path):
     lib = _lib.core()
-    b, r = C.c_int(), C.c_int()
+    b, r, m = C.c_int(), C.c_int(), C.c_int()
     lib.dslash_get_launch(3, _lib.F64, 12, C.byref(b), C.byref(r))
+    lib.dslash_get_launch_march(3, _lib.F64, 12, C.byref(m))
     saved = (b.value, r.value)
+    saved_m = m.value
     assert lib.dslash_set_launch(3, _lib.F64, 12, 192, 0) == 0
+    assert lib.dslash_set_launch_march(3, _lib.F64, 12, 8) == 0
     path = str(tmp_path / "tune.txt").encode()
     assert lib.dslash_tune_save(path) == 0
+    assert "apply 3 1 12 192 0 8\n" in open(path).read()
     assert lib.dslash_set_launch(3, _lib.F64, 12, 64, 1) == 0
+    assert lib.dslash_set_launch_march(3, _lib.F64, 12, 1) == 0
     assert lib.dslash_tune_load(path) == 0
     lib.dslash_get_launch(3, _lib.F64, 12, C.byref(b), C.byref(r))
-    assert (b.value, r.value) == (192, 0)
+    lib.dslash_get_launch_march(3, _lib.F64, 12, C.byref(m))
+    assert (b.value, r.value, m.value) == (192, 0, 8)
+    old = tmp_path / "old.txt"  # files from before the march field: march 1
+    old.write_text("apply 3 1 12 128 1\n")
+    assert lib.dslash_tune_load(str(old).encode()) == 0
+    lib.dslash_get_launch(3, _lib.F64, 12, C.byref(b), C.byref(r))
+    lib.dslash_get_launch_march(3, _lib.F64, 12, C.byref(m))
+    assert (b.value, r.value, m.value) == (128, 1, 1)
     assert lib.dslash_set_launch(3, _lib.F64, 12, *saved) == 0
+    assert lib.dslash_set_launch_march(3, _lib.F64, 12, saved_m) == 0
     bad = tmp_path / "bad.txt"
     bad.write_text("apply 4 0 18 100 1\n")
+    assert lib.dslash_tune_load(str(bad).encode()) == _lib.ERR_INVALID
+    bad.write_text("apply 4 0 18 128 1 3\n")
     assert lib.dslash_tune_load(str(bad).encode()) == _lib.ERR_INVALID
     assert lib.dslash_tune_load(b"/nonexistent/tune.txt") == _lib.ERR_INVALID
 

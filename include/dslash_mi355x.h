@@ -170,7 +170,12 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, 
  *   8  a 128-thread block walks 8 slices of 128 spatial sites together and stages
  *      each slice's spinors in a double-buffered LDS slab for the x and y hops
  *      (F32, rhs_block 1, 2, block_threads 128; launches whose problems share one
- *      isign and have nrhs divisible by rhs_block; others fall back to 4).
+ *      isign and have nrhs divisible by rhs_block; others fall back to 4);
+ *  16  the same with box tiles: a block holds whole x rows of a ty x tz box of
+ *      (y, z) (chosen per lattice to fit block_threads), so x hops and the y / z
+ *      hops inside the box read the LDS slab (F32, rhs_block 1, 2, block_threads
+ *      >= 128 and >= Lx/2 of every lattice, same launch conditions as 8; others
+ *      fall back to 8, then 4).
  * set: same arguments; the autotuner also sets them. */
 int dslash_get_mrhs_launch(int prec, int gauge_reals, int persistent, int* rhs_block, int* block_threads,
                            int* grid_blocks, int* t_block);

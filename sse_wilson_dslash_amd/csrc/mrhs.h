@@ -14,6 +14,7 @@ constexpr int MRHS_T_BLOCK = 4;      // t-slices per thread in t-marching tiles
 constexpr int MRHS_T_BLOCK_LDS = 8;  // t-slices per block in LDS-staged t-marching tiles
 constexpr int MRHS_LDS_THREADS = 128;  // threads (= spatial sites) per LDS-staged block
 constexpr int MRHS_LDS_GROUP = 8;    // RHS blocks of one spatial block run together
+constexpr int MRHS_T_BOX = 16;       // t_block value of LDS-staged box tiles (8 slices per tile)
 
 // Tile shapes: t_block 1 (one site per thread, any rhs_block), MRHS_T_BLOCK
 // (t-marching, rhs_block <= 2), MRHS_T_BLOCK_LDS (t-marching with an LDS slab,
@@ -23,6 +24,7 @@ inline bool mrhs_shape_ok(int prec, int R, int block, int t_block) {
   if (t_block == 1) return true;
   if (t_block == MRHS_T_BLOCK) return R <= 2;
   if (t_block == MRHS_T_BLOCK_LDS) return prec == F32 && R <= 2 && block == MRHS_LDS_THREADS;
+  if (t_block == MRHS_T_BOX) return prec == F32 && R <= 2 && block >= 128;
   return false;
 }
 
@@ -42,6 +44,8 @@ struct MrhsLattice {
   uint32_t rg;       // RHS blocks per group (LDS-staged tiles; nrb otherwise)
   uint32_t S;        // target sites per t-slice (Lx/2 * Ly * Lz)
   FastDiv dlxh, dly; // for the spatial coordinates of a t-slice site
+  uint32_t ty, tz;   // box tiles: a tile is Lx/2 x ty x tz sites of a slice
+  uint32_t nty;      // box tiles along y (ntz = nsb / nty along z)
 };
 
 struct MrhsList {
@@ -60,6 +64,13 @@ inline bool mrhs_lds_compatible(const dslash_mrhs_problem* probs, int nprob, int
   for (int i = 0; i < nprob; ++i)
     if (probs[i].nrhs % R != 0 || probs[i].isign != probs[0].isign) return false;
   return nprob >= 1;
+}
+// Box tiles need a whole x row of the target checkerboard in one block.
+inline bool mrhs_box_compatible(const dslash_mrhs_problem* probs, int nprob, int R, int block) {
+  if (!mrhs_lds_compatible(probs, nprob, R)) return false;
+  for (int i = 0; i < nprob; ++i)
+    if (probs[i].latt[0] / 2 > block) return false;
+  return true;
 }
 int mrhs_launch(const MrhsList& list, int prec, int G, int R, int block, int t_block, uint32_t grid, hipStream_t s);
 

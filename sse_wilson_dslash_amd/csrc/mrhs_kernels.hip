@@ -27,9 +27,11 @@
 // LDS slab; one step later the slab is slice t+1 of the source, and the x and y
 // hops of the whole block read their neighbours from it (neighbours outside the
 // block, and all z hops, load from global memory; a generic pointer selects LDS
-// or global per lane without a divergent branch).  Tiles are ordered (t-chunk,
-// group of 8 RHS blocks, spatial block, RHS block in group) so the blocks that
-// exchange z neighbours run at the same time.
+// or global per lane without a divergent branch).  The slot psi(t+1) goes into
+// held psi(t-1), the backward t neighbour, which the thread reads first, so the
+// t hops need no registers across steps.  Tiles are ordered (t-chunk, group of
+// 8 RHS blocks, spatial block, RHS block in group) so the blocks that exchange
+// z neighbours run at the same time.
 #include <hip/hip_runtime.h>
 
 #include <string.h>
@@ -214,17 +216,20 @@ __device__ void mrhs_march_lds(const MrhsLattice& L, v4u (*slab)[R][6][MRHS_LDS_
   const int xh = (int)(sc - q0 * (uint32_t)L.g.lxh);
   const uint32_t q1 = fdiv(q0, L.dly);
   const int y = (int)(q0 - q1 * (uint32_t)L.g.L[1]), z = (int)q1;
-  // preamble: slice t0 of the source at the own spatial site -> slab 0
+  // preamble: slices t0 and t0 - 1 of the source at the own spatial site -> slabs 0 and 1
   if (live) {
+    const uint32_t tm = (uint32_t)(t0 == 0 ? LT - 1 : t0 - 1);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + s + S * (uint32_t)t0;
+      const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + s;
 #pragma unroll
-      for (int v = 0; v < 6; ++v) slab[0][r][v][threadIdx.x] = f[(size_t)v * vh];
+      for (int v = 0; v < 6; ++v) {
+        slab[0][r][v][threadIdx.x] = f[(size_t)v * vh + S * (uint32_t)t0];
+        slab[1][r][v][threadIdx.x] = f[(size_t)v * vh + S * tm];
+      }
     }
   }
   __syncthreads();
-  Half<float> fifo[2][R];
 #pragma unroll
   for (int k = 0; k < TB; ++k) {
     const int t = t0 + k;
@@ -241,26 +246,24 @@ __device__ void mrhs_march_lds(const MrhsLattice& L, v4u (*slab)[R][6][MRHS_LDS_
         for (int sp = 0; sp < 4; ++sp)
 #pragma unroll
           for (int cc = 0; cc < 3; ++cc) acc[r].s[sp][cc] = {0.f, 0.f};
-      {  // t hops: the forward neighbour also fills the slab of the next step
+      {  // t hops.  The own slot of the other slab holds slice t - 1 (the
+         // backward neighbour); it is read, then overwritten with slice t + 1
+         // (the forward neighbour), which is the next step's slab.  No other
+         // lane reads that slab during this step.
         C<float> Wf[3][3], Wb[3][3];
         load_pair<F32, G, false>(L.gauge + (size_t)3 * PV * vh, idx, vh, Wf, Wb);
         const uint32_t nf = s + S * (uint32_t)(t + 1 == LT ? 0 : t + 1);
-        const uint32_t nb = s + S * (uint32_t)(t == 0 ? LT - 1 : t - 1);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride;
+          const Spinor<float> q = spinor_at(&slab[cur ^ 1][r][0][threadIdx.x], (size_t)BS);
+          spin_recon_add<3, IS>(acc[r], su3_mul(Wb, spin_project<3, IS>(q)));
           v4u buf[6];
 #pragma unroll
           for (int v = 0; v < 6; ++v) buf[v] = f[(size_t)v * vh + nf];
 #pragma unroll
           for (int v = 0; v < 6; ++v) slab[cur ^ 1][r][v][threadIdx.x] = buf[v];
-          const Spinor<float> pf = spinor_of(buf);
-          spin_recon_add<3, -IS>(acc[r], su3_mul(Wf, spin_project<3, -IS>(pf)));
-          Half<float> hb;
-          if (k < 2) hb = spin_project<3, IS>(load_spinor<F32>(f, nullptr, nb, vh));
-          else hb = fifo[k & 1][r];
-          spin_recon_add<3, IS>(acc[r], su3_mul(Wb, hb));
-          if (k + 2 < TB) fifo[k & 1][r] = spin_project<3, IS>(pf);
+          spin_recon_add<3, -IS>(acc[r], su3_mul(Wf, spin_project<3, -IS>(spinor_of(buf))));
         }
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -298,6 +301,179 @@ __global__ void __launch_bounds__(MRHS_LDS_THREADS) k_mrhs_lds(MrhsList list) {
     const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
     const int r0 = (int)(grp * L.rg + rbi) * R;
     mrhs_march_lds<G, IS, R>(L, slab, sb * MRHS_LDS_THREADS, (int)tc * MRHS_T_BLOCK_LDS, r0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged box tiles.  A block owns a box of Lx/2 x ty x tz spatial sites (a
+// whole x row, ty rows in y, tz planes in z) of one (t-chunk, RHS block) and
+// walks 8 slices together like the LDS-staged tiles above.  Every x neighbour
+// and every y / z neighbour inside the box comes from the LDS slab; only the
+// y / z hops that leave the box load from global memory.  With the contiguous
+// 128-site blocks all z hops went to global memory, and at 32 fields in flight
+// L2 (4 MB per XCD) turns over faster than the neighbour blocks reach the same
+// slice, so those loads were re-fetched from past L2 (3.7 GB per launch at
+// 32 RHS on 24^3 x 48 against 1.2 GB compulsory reads).
+// Slab layout (dynamic LDS): [2 buffers][R][6 vectors][blockDim.x lanes].
+// ---------------------------------------------------------------------------
+struct BoxLane {
+  int i;           // x row position (= x / 2)
+  int y, z;        // lattice coordinates
+  int y0, z0;      // the box's first y and z (uniform over the block)
+  bool live;       // a site of the lattice (partial boxes at the y / z ends)
+};
+
+template <int G, int IS, int R, int MU>
+__device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, const BoxLane& b, const Coord& c,
+                                        uint32_t idx, uint32_t tS, int r0, Spinor<float> (&acc)[R]) {
+  constexpr int PV = PairVec<F32, G>::value;
+  const uint32_t vh = L.g.vh, nt = blockDim.x;
+  const int lxh = L.g.lxh;
+  C<float> Wf[3][3], Wb[3][3];
+  load_pair<F32, G, false>(L.gauge + (size_t)MU * PV * vh, idx, vh, Wf, Wb);
+  const Coord cf = shift_coord(L.g, c, MU, +1), cb = shift_coord(L.g, c, MU, -1);
+  // neighbour lanes inside the box (x: always; y / z: when the shifted
+  // coordinate stays in the box's range, wrap included)
+  int lf, lb;
+  bool inf, inb;
+  const int lane = (int)threadIdx.x;
+  if constexpr (MU == 0) {
+    lf = lane - b.i + (cf.x >> 1);
+    lb = lane - b.i + (cb.x >> 1);
+    inf = inb = true;
+  } else if constexpr (MU == 1) {
+    inf = (unsigned)(cf.y - b.y0) < L.ty;
+    inb = (unsigned)(cb.y - b.y0) < L.ty;
+    lf = lane + (cf.y - b.y) * lxh;
+    lb = lane + (cb.y - b.y) * lxh;
+  } else {
+    inf = (unsigned)(cf.z - b.z0) < L.tz;
+    inb = (unsigned)(cb.z - b.z0) < L.tz;
+    lf = lane + (cf.z - b.z) * lxh * (int)L.ty;
+    lb = lane + (cb.z - b.z) * lxh * (int)L.ty;
+  }
+  const uint32_t nf = site_index<4>(L.g, cf) - tS, nb = site_index<4>(L.g, cb) - tS;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + tS;
+    const v4u* sr = slab + (size_t)r * 6 * nt;
+    const v4u* pf = inf ? sr + lf : f + nf;
+    const v4u* pb = inb ? sr + lb : f + nb;
+    const Spinor<float> p = spinor_at(pf, inf ? (size_t)nt : (size_t)vh);
+    const Spinor<float> q = spinor_at(pb, inb ? (size_t)nt : (size_t)vh);
+    spin_recon_add<MU, -IS>(acc[r], su3_mul(Wf, spin_project<MU, -IS>(p)));
+    spin_recon_add<MU, IS>(acc[r], su3_mul(Wb, spin_project<MU, IS>(q)));
+  }
+}
+
+// One box tile: box (bty, btz) of slices t0 .. t0+7 (clipped to T), fields
+// r0 .. r0+R-1.  Every thread executes the same barriers.
+template <int G, int IS, int R>
+__device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0, int r0) {
+  constexpr int TB = MRHS_T_BLOCK_LDS;
+  constexpr int PV = PairVec<F32, G>::value;
+  const uint32_t vh = L.g.vh, S = L.S, nt = blockDim.x;
+  const int LT = L.g.L[3], lxh = L.g.lxh;
+  BoxLane b;
+  {
+    const int lane = (int)threadIdx.x, rest = lane / lxh;
+    const int j = rest % (int)L.ty, k = rest / (int)L.ty;
+    b.i = lane - rest * lxh;
+    b.y0 = (int)(bty * L.ty);
+    b.z0 = (int)(btz * L.tz);
+    b.y = b.y0 + j;
+    b.z = b.z0 + k;
+    b.live = k < (int)L.tz && b.y < L.g.L[1] && b.z < L.g.L[2];
+  }
+  const uint32_t s = (uint32_t)b.i + (uint32_t)lxh * ((uint32_t)b.y + (uint32_t)L.g.L[1] * (uint32_t)b.z);
+  const size_t bufsz = (size_t)R * 6 * nt;
+  if (b.live) {  // preamble: slice t0 at the own site -> buffer 0, slice t0 - 1 -> buffer 1
+    const uint32_t tm = (uint32_t)(t0 == 0 ? LT - 1 : t0 - 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + s;
+#pragma unroll
+      for (int v = 0; v < 6; ++v) {
+        slab[((size_t)r * 6 + v) * nt + threadIdx.x] = f[(size_t)v * vh + S * (uint32_t)t0];
+        slab[bufsz + ((size_t)r * 6 + v) * nt + threadIdx.x] = f[(size_t)v * vh + S * tm];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < TB; ++k) {
+    const int t = t0 + k;
+    if (t >= LT) break;
+    const int cur = k & 1;
+    if (b.live) {
+      const Coord c{2 * b.i + ((L.tcb + b.y + b.z + t) & 1), b.y, b.z, t};
+      const uint32_t tS = S * (uint32_t)t;
+      const uint32_t idx = s + tS;
+      Spinor<float> acc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) acc[r].s[sp][cc] = {0.f, 0.f};
+      {  // t hops.  The own slot of the other buffer holds slice t - 1 (the
+         // backward neighbour); it is read, then overwritten with slice t + 1
+         // (the forward neighbour), which is the next step's slab.  No other
+         // lane reads that buffer during this step.
+        C<float> Wf[3][3], Wb[3][3];
+        load_pair<F32, G, false>(L.gauge + (size_t)3 * PV * vh, idx, vh, Wf, Wb);
+        const uint32_t nf = s + S * (uint32_t)(t + 1 == LT ? 0 : t + 1);
+        v4u* other = slab + (size_t)(cur ^ 1) * bufsz;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride;
+          const Spinor<float> q = spinor_at(other + (size_t)r * 6 * nt + threadIdx.x, nt);
+          spin_recon_add<3, IS>(acc[r], su3_mul(Wb, spin_project<3, IS>(q)));
+          v4u buf[6];
+#pragma unroll
+          for (int v = 0; v < 6; ++v) buf[v] = f[(size_t)v * vh + nf];
+#pragma unroll
+          for (int v = 0; v < 6; ++v) other[((size_t)r * 6 + v) * nt + threadIdx.x] = buf[v];
+          spin_recon_add<3, -IS>(acc[r], su3_mul(Wf, spin_project<3, -IS>(spinor_of(buf))));
+        }
+      }
+      const v4u* here = slab + (size_t)cur * bufsz;
+      __builtin_amdgcn_sched_barrier(0);
+      box_hop<G, IS, R, 0>(L, here, b, c, idx, tS, r0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      box_hop<G, IS, R, 1>(L, here, b, c, idx, tS, r0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      box_hop<G, IS, R, 2>(L, here, b, c, idx, tS, r0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) store_spinor<F32, true>(L.res + (size_t)(r0 + r) * L.fstride, nullptr, idx, vh, acc[r]);
+    }
+    __syncthreads();  // buffer cur ^ 1 complete; buffer cur free for the next step's writes
+  }
+  __syncthreads();  // the next tile's preamble overwrites buffer 0
+}
+
+template <int G, int R, int IS>
+__global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list) {
+  extern __shared__ v4u box_slab[];
+  const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
+  const uint32_t total = list.total_tiles;
+  const uint32_t x = blockIdx.x % NX;
+  const uint32_t per_x = gridDim.x / NX + (x < (gridDim.x % NX) ? 1u : 0u);
+  const uint32_t j = blockIdx.x / NX;
+  const uint32_t t_lo = (uint32_t)((uint64_t)total * x / NX), t_hi = (uint32_t)((uint64_t)total * (x + 1) / NX);
+  for (uint32_t t = t_lo + j; t < t_hi; t += per_x) {
+    int l = 0;
+    while (l + 1 < list.n && t >= list.lat[l + 1].tile0) ++l;
+    const MrhsLattice& L = list.lat[l];
+    // local tile = ((tc * ngroups + group) * nboxes + box) * rg + rb_in_group,
+    // box = bty + nty * btz
+    const uint32_t local = t - L.tile0;
+    const uint32_t q = local / L.rg, rbi = local - q * L.rg;
+    const uint32_t q2 = q / L.nsb, box = q - q2 * L.nsb;
+    const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
+    const uint32_t btz = box / L.nty, bty = box - btz * L.nty;
+    mrhs_box_tile<G, IS, R>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS, (int)(grp * L.rg + rbi) * R);
   }
 }
 
@@ -348,6 +524,18 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
   } else if (TB == MRHS_T_BLOCK) {
     if (R == 1) hipLaunchKernelGGL((k_mrhs<P, G, 1, MRHS_T_BLOCK>), dim3(grid), dim3(block), 0, s, list);
     else hipLaunchKernelGGL((k_mrhs<P, G, 2, MRHS_T_BLOCK>), dim3(grid), dim3(block), 0, s, list);
+  } else if (TB == MRHS_T_BOX) {
+    if constexpr (P == F32) {  // validated by mrhs_shape_ok / mrhs_box_compatible; one isign per list
+      const bool plus = list.lat[0].isign > 0;
+      const size_t lds = (size_t)2 * R * 6 * block * sizeof(v4u);
+      if (R == 1) {
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 1, 1>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box<G, 1, -1>), dim3(grid), dim3(block), lds, s, list);
+      } else {
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 2, 1>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box<G, 2, -1>), dim3(grid), dim3(block), lds, s, list);
+      }
+    }
   } else if constexpr (P == F32) {  // MRHS_T_BLOCK_LDS (validated by mrhs_shape_ok; one isign per list)
     const bool plus = list.lat[0].isign > 0;
     if (R == 1) {
@@ -360,13 +548,37 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
   }
 }
 
+// Box extents (ty, tz) for a block of nt threads: divisors of Ly and Lz (no
+// partial boxes) with lxh * ty * tz <= nt, chosen to minimise the global loads
+// per site: 1/ty + 1/tz hops leave the box (0 for a direction the box spans),
+// plus the site's other traffic (taken as 0.5 hop), over the fraction of the
+// block's lanes that hold a site.
+static void box_shape(int lxh, int ly, int lz, int nt, uint32_t* ty, uint32_t* tz) {
+  double best = 1e30;
+  *ty = *tz = 1;
+  for (int a = 1; a <= ly; ++a) {
+    if (ly % a) continue;
+    for (int c = 1; c <= lz; ++c) {
+      if (lz % c || lxh * a * c > nt) continue;
+      const double out = (a < ly ? 1.0 / a : 0.0) + (c < lz ? 1.0 / c : 0.0);
+      const double cost = (out + 0.5) * nt / (double)(lxh * a * c);
+      if (cost < best - 1e-12) {
+        best = cost;
+        *ty = (uint32_t)a;
+        *tz = (uint32_t)c;
+      }
+    }
+  }
+}
+
 int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R, int block, int t_block,
                     MrhsList* list) {
   WD_REQUIRE(probs && nprob >= 1 && nprob <= MRHS_MAX_LATTICES, "1..%d problems per launch", MRHS_MAX_LATTICES);
   WD_REQUIRE(mrhs_shape_ok(prec, R, block, t_block),
              "unsupported batched launch shape (rhs_block %d, %d threads, t_block %d): t_block 1 takes rhs_block "
-             "1/2/4; t_block %d takes rhs_block 1/2; t_block %d takes F32, rhs_block 1/2 and %d threads",
-             R, block, t_block, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS);
+             "1/2/4; t_block %d takes rhs_block 1/2; t_block %d takes F32, rhs_block 1/2 and %d threads; "
+             "t_block %d takes F32, rhs_block 1/2 and >= 128 threads",
+             R, block, t_block, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS, MRHS_T_BOX);
   WD_REQUIRE(block >= 64 && block <= 256 && block % 64 == 0, "block_threads must be 64..256, multiple of 64");
   memset(list, 0, sizeof(*list));
   uint64_t tiles = 0;
@@ -396,13 +608,20 @@ int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R
     L.S = (uint32_t)(L.g.lxh * p.latt[1] * p.latt[2]);
     L.dlxh = make_fastdiv(L.g.lxh);
     L.dly = make_fastdiv(p.latt[1]);
-    WD_REQUIRE(t_block != MRHS_T_BLOCK_LDS || (p.nrhs % R == 0 && p.isign == probs[0].isign),
+    WD_REQUIRE((t_block != MRHS_T_BLOCK_LDS && t_block != MRHS_T_BOX) || (p.nrhs % R == 0 && p.isign == probs[0].isign),
                "problem %d: LDS-staged tiles need nrhs (%d) to be a multiple of rhs_block (%d) and one isign "
                "for the whole launch", i, p.nrhs, R);
     L.rg = L.nrb;
     if (t_block == 1) {
       L.nsb = (L.g.vh + block - 1) / block;
       L.ntc = 1;
+    } else if (t_block == MRHS_T_BOX) {
+      WD_REQUIRE(L.g.lxh <= block, "problem %d: box tiles need Lx/2 (%d) <= block_threads (%d)", i, L.g.lxh, block);
+      box_shape(L.g.lxh, p.latt[1], p.latt[2], block, &L.ty, &L.tz);
+      L.nty = (p.latt[1] + L.ty - 1) / L.ty;
+      L.nsb = L.nty * ((p.latt[2] + L.tz - 1) / L.tz);
+      L.ntc = (p.latt[3] + MRHS_T_BLOCK_LDS - 1) / MRHS_T_BLOCK_LDS;
+      if (L.nrb > MRHS_LDS_GROUP && L.nrb % MRHS_LDS_GROUP == 0) L.rg = MRHS_LDS_GROUP;
     } else {
       L.nsb = (L.S + block - 1) / block;
       L.ntc = (p.latt[3] + t_block - 1) / t_block;

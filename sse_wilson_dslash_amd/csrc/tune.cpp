@@ -24,13 +24,14 @@ struct MrhsCfg {
   int R;
   int block;
   int grid;  // persistent grid blocks (0 for the one-tile-per-block launch)
-  int tb;    // tile shape: 1, MRHS_T_BLOCK (t-marching) or MRHS_T_BLOCK_LDS (t-marching with an LDS slab)
+  int tb;    // tile shape: 1, MRHS_T_BLOCK (t-marching), MRHS_T_BLOCK_LDS (t-marching with an LDS
+             // slab) or MRHS_T_BOX (the same with box tiles)
 };
 
 // [format F32/F64][G==12][persistent]; defaults measured at 32 RHS on 24^3 x 48
 static MrhsCfg g_mrhs[2][2][2] = {
-    {{{2, MRHS_LDS_THREADS, 0, MRHS_T_BLOCK_LDS}, {1, MRHS_LDS_THREADS, 1024, MRHS_T_BLOCK_LDS}},
-     {{2, MRHS_LDS_THREADS, 0, MRHS_T_BLOCK_LDS}, {1, MRHS_LDS_THREADS, 1024, MRHS_T_BLOCK_LDS}}},
+    {{{2, 192, 0, MRHS_T_BOX}, {1, MRHS_LDS_THREADS, 1024, MRHS_T_BLOCK_LDS}},
+     {{2, 192, 0, MRHS_T_BOX}, {1, MRHS_LDS_THREADS, 1024, MRHS_T_BLOCK_LDS}}},
     {{{2, 128, 0, 1}, {2, 128, 2048, 1}}, {{2, 128, 0, 1}, {2, 128, 2048, 1}}}};
 
 static MrhsCfg& mrhs_entry(int prec, int G, int persistent) {
@@ -82,6 +83,9 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int
   int block = block_threads ? block_threads : c.block;
   // the tabled tile shape, or the closest one this launch supports
   int tb = c.tb;
+  if (tb == MRHS_T_BOX && (!mrhs_shape_ok(prec, R, block, tb) || !probs || nprob < 1 ||
+                           !mrhs_box_compatible(probs, nprob, R, block)))
+    tb = MRHS_T_BLOCK_LDS;
   if (tb == MRHS_T_BLOCK_LDS && (!mrhs_shape_ok(prec, R, block, tb) || !probs || nprob < 1 ||
                                  !mrhs_lds_compatible(probs, nprob, R)))
     tb = MRHS_T_BLOCK;
@@ -111,8 +115,8 @@ int dslash_set_mrhs_launch(int prec, int G, int persistent, int rhs_block, int b
   WD_REQUIRE(block_threads >= 64 && block_threads <= 256 && block_threads % 64 == 0,
              "block_threads must be 64, 128, 192 or 256");
   WD_REQUIRE(mrhs_shape_ok(prec, rhs_block, block_threads, t_block),
-             "t_block must be 1, %d (rhs_block 1 or 2) or %d (F32, rhs_block 1 or 2, %d threads)", MRHS_T_BLOCK,
-             MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS);
+             "t_block must be 1, %d (rhs_block 1 or 2), %d (F32, rhs_block 1 or 2, %d threads) or %d (F32, "
+             "rhs_block 1 or 2, >= 128 threads)", MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS, MRHS_T_BOX);
   WD_REQUIRE(persistent ? grid_blocks > 0 : grid_blocks == 0, "grid_blocks must be > 0 exactly when persistent");
   mrhs_entry(prec, G, persistent) = MrhsCfg{rhs_block, block_threads, grid_blocks, t_block};
   return 0;
@@ -183,12 +187,14 @@ int dslash_tune_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int 
   const int grids_np[] = {0};
   const int* grids = persistent ? grids_p : grids_np;
   const int ng = persistent ? 4 : 1;
-  for (int tb : {1, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS})
+  for (int tb : {1, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_T_BOX})
     for (int R : {1, 2, 4}) {
       if (tb == MRHS_T_BLOCK_LDS && !mrhs_lds_compatible(probs, nprob, R)) continue;
-      for (int block : {64, 128, 256})
+      for (int block : {64, 128, 192, 256})
         for (int gi = 0; gi < ng; ++gi) {
           if (!mrhs_shape_ok(prec, R, block, tb)) continue;
+          if (tb == MRHS_T_BOX && !mrhs_box_compatible(probs, nprob, R, block)) continue;
+          if (tb != MRHS_T_BOX && block == 192) continue;
           float t = 0;
           int grid = grids[gi];
           int e = time_it(s, 10, [&]() { return mrhs_run(probs, nprob, prec, G, R, block, grid, tb, s); }, &t);

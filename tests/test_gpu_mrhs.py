@@ -120,6 +120,53 @@ def test_lds_tiles_persistent_and_fallback(oracle, launch_table):
         check_item(oracle, item, u, host, 1e-5, ks=[0, 3])
 
 
+@pytest.mark.parametrize("rhs_block,nrhs", [(1, 7), (2, 8)])
+@pytest.mark.parametrize("block", [128, 192, 256])
+@pytest.mark.parametrize("latt", [(8, 8, 8, 16), (8, 4, 6, 10), (12, 6, 4, 8), (24, 6, 10, 4)])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_box_tiles(oracle, launch_table, rhs_block, nrhs, block, latt, isign):
+    """t_block 16: box tiles of whole x rows x ty x tz (boxes spanning all of y or
+    z wrap inside the box; T = 10 leaves a 2-slice chunk; 6, 12 and 18 x rows
+    leave idle lanes in the block) against the single-RHS kernel and the oracle."""
+    print(f"[box tiles] latt={latt} block={block} R={rhs_block} isign={isign}", flush=True)
+    item, u, host = make_item(latt, nrhs, isign=isign, target_cb=1)
+    set_launch(F32, 18, False, rhs_block, block, 0, 16)
+    apply_batch([item], rhs_block=rhs_block, block_threads=block)
+    torch.cuda.synchronize()
+    op = item.op
+    for k in range(nrhs):
+        single = op.new_spinor(False)
+        op.apply_cb(single, item.psi.field(k), isign, 1)
+        a = op.unpack_cb(single, 1, dtype=np.float32)
+        b = op.unpack_cb(item.res.field(k), 1, dtype=np.float32)
+        assert R.rel_err(b, a) < 1e-6, (k, R.rel_err(b, a))
+    check_item(oracle, item, u, host, 1e-5, ks=[0, nrhs - 1])
+
+
+def test_box_tiles_persistent_and_fallback(oracle, launch_table):
+    """A persistent box-tile launch over three lattices (4, 8 and 12 sites per x
+    row) with one isign; a lattice with Lx/2 > block_threads falls back to the
+    LDS tiles, mixed isign to the register march."""
+    items = [make_item(latt, 4, F32, 18, -1, 0, seed)
+             for latt, seed in (((8, 8, 8, 8), 6), ((16, 16, 16, 8), 7), ((24, 24, 24, 8), 8))]
+    set_launch(F32, 18, True, 2, 192, 96, 16)
+    apply_batch([it[0] for it in items], grid_blocks=96)
+    torch.cuda.synchronize()
+    for item, u, host in items:
+        check_item(oracle, item, u, host, 1e-5)
+    wide = [make_item((264, 2, 2, 4), 2, F32, 18, 1, 0, 9)]
+    set_launch(F32, 18, False, 2, 128, 0, 16)
+    apply_batch([it[0] for it in wide])
+    torch.cuda.synchronize()
+    for item, u, host in wide:
+        check_item(oracle, item, u, host, 1e-5)
+    mixed = uneven_items()
+    apply_batch([it[0] for it in mixed])
+    torch.cuda.synchronize()
+    for item, u, host in mixed:
+        check_item(oracle, item, u, host, 1e-5, ks=[0, 3])
+
+
 def uneven_items():
     out = []
     for latt, isign, tcb, seed in (((8, 8, 8, 8), 1, 0, 3), ((16, 16, 16, 32), -1, 1, 4), ((24, 24, 24, 48), 1, 1, 5)):
@@ -182,8 +229,8 @@ def test_autotuner(oracle):
     its = [it[0] for it in items]
     for persistent in (False, True):
         r = tune_batch(its, persistent)
-        assert r["rhs_block"] in (1, 2, 4) and r["block_threads"] in (64, 128, 256) and r["ms"] > 0
-        assert r["t_block"] in (1, 4) and not (r["t_block"] == 4 and r["rhs_block"] == 4)
+        assert r["rhs_block"] in (1, 2, 4) and r["block_threads"] in (64, 128, 192, 256) and r["ms"] > 0
+        assert r["t_block"] in (1, 4, 8, 16) and not (r["t_block"] > 1 and r["rhs_block"] == 4)
         assert (r["grid_blocks"] > 0) == persistent
         for it in its:
             it.res.buf.zero_()

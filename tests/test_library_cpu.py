@@ -257,6 +257,10 @@ def test_mrhs_launch_table_and_tune_file(tmp_path):
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 192, 0, 3) == _lib.ERR_INVALID
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 64, 0, 8) == _lib.ERR_INVALID  # LDS tiles: 128 threads
     assert lib.dslash_set_mrhs_launch(_lib.F64, 18, 0, 2, 128, 0, 8) == _lib.ERR_INVALID  # LDS tiles: F32 only
+    assert lib.dslash_set_mrhs_launch(_lib.F64, 18, 0, 2, 192, 0, 16) == _lib.ERR_INVALID  # box tiles: F32 only
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 4, 192, 0, 16) == _lib.ERR_INVALID  # rhs_block <= 2
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 64, 0, 16) == _lib.ERR_INVALID  # >= 128 threads
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 192, 0, 16) == 0
     assert lib.dslash_set_mrhs_launch(_lib.F32, 12, 1, 2, 128, 1024, 8) == 0
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 192, 0, 4) == 0
     path = str(tmp_path / "tune.txt")

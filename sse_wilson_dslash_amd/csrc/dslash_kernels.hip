@@ -57,7 +57,8 @@ void clear_error() {
 // link reals); set by dslash_set_launch / dslash_set_launch_march or by the
 // autotuner.  Defaults are the shapes measured fastest at 32^3 x 64 on MI355X
 // (XCD-contiguous order is worth ~25%; marching 6-16% for FP64, sloppy FP64 and
-// the 3D operator, ~1% for 4D FP32, nothing for 16-bit spinors).
+// the 3D operator; for 4D FP32 it is within 1% at 32^3 x 64 and 3-5% slower at
+// 48^3 x 96, and it gains nothing for 16-bit spinors).
 // Index: [nd == 4][F32, F64, F16, BF16, F64S][G == 12].
 static LaunchCfg g_cfg[2][NUM_PREC][2] = {
     {{{256, 1, 8}, {256, 1, 8}},
@@ -65,8 +66,8 @@ static LaunchCfg g_cfg[2][NUM_PREC][2] = {
      {{64, 1, 1}, {64, 1, 8}},
      {{64, 1, 1}, {64, 1, 1}},
      {{256, 1, 8}, {256, 1, 8}}},
-    {{{256, 1, 8}, {256, 1, 8}},
-     {{256, 1, 8}, {64, 1, 8}},
+    {{{64, 1, 1}, {64, 1, 1}},
+     {{256, 1, 8}, {256, 1, 8}},
      {{256, 1, 1}, {256, 1, 1}},
      {{256, 1, 1}, {256, 1, 1}},
      {{256, 1, 8}, {256, 1, 8}}}};

@@ -21,6 +21,7 @@
 #include "dslash_mi355x.h"
 #include "sse_dslash.h"
 #include "sse_dslash_3d.h"
+#include "stager.h"
 
 #if SSE_PRECISION == 32
 #define LEG_PREC DSLASH_F32
@@ -61,6 +62,7 @@ struct Legacy {
   int prec = LEG_DEV_PREC;   // device format of bundles and kernel (LEG_PREC or F64_SLOPPY)
   hipStream_t stream = nullptr;
   std::vector<SSEREAL> h_stage;
+  wd::Stager stager;
 
   explicit Legacy(int nd_, const char* name_) : nd(nd_), name(name_) {}
 
@@ -179,6 +181,7 @@ struct Legacy {
       }
       h_stage.resize(Vh * 24);
     }
+    stager.start();
     resident_G = 0;
     refcount = 1;
     last_error = 0;
@@ -188,6 +191,7 @@ struct Legacy {
     if (refcount == 0) return;  // lib/sse_su3dslash_32bit_scalar.c:127-129
     if (--refcount > 0) return;
     if (stream) (void)hipStreamSynchronize(stream);
+    stager.finish();
     void* ptrs[] = {d_u_aos, d_psi_aos, d_res_aos, d_psi_cb, d_res_cb, d_bundle, d_bundle_call, d_site_table};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
@@ -211,7 +215,7 @@ struct Legacy {
 
   // Upload the packed gauge and build the bundles of the listed target cbs.
   int upload_gauge(const SSEREAL* u, int G, void* const slot[2]) {
-    hipError_t e = hipMemcpyAsync(d_u_aos, u, V * 4 * 18 * RS, hipMemcpyHostToDevice, stream);
+    hipError_t e = stager.h2d(d_u_aos, u, V * 4 * 18 * RS, stream);
     if (e != hipSuccess) return hip_fail(e, "gauge upload");
     for (int t = 0; t < 2; ++t)
       if (slot[t] && dslash_pack_gauge(slot[t], t, d_u_aos, LEG_PREC, latt, nd, prec, G, d_site_table, stream))
@@ -224,10 +228,10 @@ struct Legacy {
     const int tcb = 1 - cb;
     hipError_t e;
     if (canonical)
-      e = hipMemcpyAsync((char*)d_psi_aos + (size_t)cb * Vh * 24 * RS, psi + (size_t)cb * Vh * 24, Vh * 24 * RS,
-                         hipMemcpyHostToDevice, stream);
+      e = stager.h2d((char*)d_psi_aos + (size_t)cb * Vh * 24 * RS, psi + (size_t)cb * Vh * 24, Vh * 24 * RS,
+                     stream);
     else
-      e = hipMemcpyAsync(d_psi_aos, psi, V * 24 * RS, hipMemcpyHostToDevice, stream);
+      e = stager.h2d(d_psi_aos, psi, V * 24 * RS, stream);
     if (e != hipSuccess) return hip_fail(e, "spinor upload");
     if (dslash_pack_spinor(d_psi_cb, cb, d_psi_aos, LEG_PREC, latt, nd, prec, d_site_table, stream))
       return fail(dslash_last_error(), "spinor pack");
@@ -236,16 +240,14 @@ struct Legacy {
     if (canonical) {
       if (dslash_unpack_spinor(d_res_aos, tcb, d_res_cb, prec, latt, nd, LEG_PREC, nullptr, 0, stream))
         return fail(dslash_last_error(), "spinor unpack");
-      e = hipMemcpyAsync(res + (size_t)tcb * Vh * 24, (char*)d_res_aos + (size_t)tcb * Vh * 24 * RS, Vh * 24 * RS,
-                         hipMemcpyDeviceToHost, stream);
+      e = stager.d2h(res + (size_t)tcb * Vh * 24, (char*)d_res_aos + (size_t)tcb * Vh * 24 * RS, Vh * 24 * RS,
+                     stream);
       if (e != hipSuccess) return hip_fail(e, "result download");
-      if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "synchronize");
     } else {
       if (dslash_unpack_spinor(d_res_aos, tcb, d_res_cb, prec, latt, nd, LEG_PREC, nullptr, 1, stream))
         return fail(dslash_last_error(), "spinor unpack");
-      e = hipMemcpyAsync(h_stage.data(), d_res_aos, Vh * 24 * RS, hipMemcpyDeviceToHost, stream);
+      e = stager.d2h(h_stage.data(), d_res_aos, Vh * 24 * RS, stream);
       if (e != hipSuccess) return hip_fail(e, "result download");
-      if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "synchronize");
       for (size_t i = 0; i < Vh; ++i)
         memcpy(res + (size_t)site_table[(size_t)tcb * Vh + i] * 24, &h_stage[i * 24], 24 * RS);
     }

@@ -136,6 +136,50 @@ def test_legacy_caller_order_callbacks(oracle, libs, nd):
         free()
 
 
+@pytest.mark.parametrize("prec,nd,latt,order", [(32, 4, (32, 32, 32, 66), "canonical"),
+                                                (64, 3, (32, 32, 34, 33), "canonical"),
+                                                (64, 4, (16, 16, 40, 36), "permuted")])
+def test_legacy_staged_transfers(oracle, libs, prec, nd, latt, order):
+    """Host arrays of 64 MiB and more move through the pinned double-buffered
+    staging (32 MiB chunks; these sizes leave a partial last chunk: gauge
+    594 / 631 / 202.5 MiB, source half 99 / 105 MiB, permuted full source
+    67.5 MiB; the permuted case's 34 MiB result takes the direct copy)."""
+    lg = libs[prec]
+    dt = lg.dtype
+    V = volume(latt)
+    print(f"[legacy staged] prec={prec} nd={nd} latt={latt} {order}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=4, dtype=dt)
+    psi = gen_spinor_host(latt, nd, seed=4, dtype=dt)
+    init = lg.init_sse_su3dslash if nd == 4 else lg.init_sse_su3dslash_3d
+    free = lg.free_sse_su3dslash if nd == 4 else lg.free_sse_su3dslash_3d
+    apply = lg.sse_su3dslash_wilson if nd == 4 else lg.sse_su3dslash_wilson_3d
+    if order == "permuted":
+        perm = np.random.default_rng(6).permutation(V)
+        init(latt, *table_callbacks(latt, perm, nd))
+        u_c = np.empty_like(u); u_c[perm] = u
+        p_c = np.empty_like(psi); p_c[perm] = psi
+    else:
+        perm = None
+        init(latt)
+        u_c, p_c = u, psi
+    try:
+        vh = V // 2
+        for isign, cb in ((1, 0), (-1, 1)):
+            res = np.full_like(psi, -3.0)
+            apply(u_c, p_c, res, isign, cb)
+            if perm is not None:
+                res = res[perm]
+            want = oracle.dslash(u, psi, np.full_like(psi, -3.0), latt, isign, cb, nd=nd)
+            e = R.rel_err(res, want)
+            print(f"  isign={isign:+d} cb={cb}: rel L2 {e:.2e}", flush=True)
+            assert e < (1e-5 if prec == 32 else 1e-13)
+            assert R.max_rel_err(res, want) < (1e-5 if prec == 32 else 1e-13)
+            assert np.all(res.reshape(-1, 24)[cb * vh:(cb + 1) * vh] == -3.0)
+        assert (lg.last_error() if nd == 4 else lg.last_error_3d()) == 0
+    finally:
+        free()
+
+
 def test_legacy_semantics(oracle, libs):
     """Aliasing psi == res, isign no-op, refcounting, resident-gauge paths."""
     lg = libs[32]

@@ -327,6 +327,20 @@ def test_qdp_packer_headers(tmp_path, prec):
     assert out.returncode == 0 and out.stdout.startswith("OK"), out.stdout + out.stderr
 
 
+@pytest.mark.parametrize("sanitizer", ["thread", "address"])
+def test_staging_pool(tmp_path, sanitizer):
+    """The drop-in API's pinned staging (csrc/stager.h) on the CPU through a
+    HIP stub: byte-exact round trips across the pipelining threshold and
+    restarted pools, under ThreadSanitizer and AddressSanitizer."""
+    exe = tmp_path / f"stager_{sanitizer}"
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-Wall", "-Werror", f"-fsanitize={sanitizer}",
+           "-I", os.path.join(ROOT, "tests", "hip_stub"), "-I", os.path.join(ROOT, "sse_wilson_dslash_amd", "csrc"),
+           "-o", str(exe), os.path.join(ROOT, "tests", "stager_check.cpp"), "-lpthread"]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=180)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0 and out.stdout.startswith("stager ok"), out.stdout + out.stderr[-2000:]
+
+
 @pytest.mark.parametrize("prec", [32, 64])
 def test_c_gauge_packer(prec):
     """pack_gauge_field reorders [mu][cb][Vh] -> [cb][Vh][mu] without transposing

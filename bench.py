@@ -316,6 +316,40 @@ def cg_leg(latt=(48, 48, 48, 96), kappa=0.12, tol=1e-10):
     return out
 
 
+def mrhs_pmc_workload(warmup: int = 8, reps: int = 30):
+    """For rocprofv3 --pmc runs: the 32-RHS 24^3x48 launch with the two LDS-staged
+    shapes fixed (128-site rows, then 12x4x4 boxes at 192 threads), each warmup +
+    reps launches, no tuning."""
+    import torch
+    from sse_wilson_dslash_amd import F32
+    from sse_wilson_dslash_amd.batch import BatchItem, RhsBatch, apply_batch, get_launch, set_launch
+    from sse_wilson_dslash_amd.device import WilsonDslash
+
+    op = WilsonDslash((24, 24, 24, 48), 4, F32, 18)
+    op.gen_gauge(seed=1)
+    psi = RhsBatch(op, 32)
+    for k in range(32):
+        psi.field(k).copy_(op.cb_view(op.gen_spinor(seed=1, field=k), 1))
+    items = [BatchItem(op, psi, RhsBatch(op, 32, zero=True), 1, 0)]
+    saved = get_launch(F32, 18, False)
+    try:
+        for R, block, t_block in ((2, 128, 8), (2, 192, 16)):
+            set_launch(F32, 18, False, R, block, 0, t_block)
+            for _ in range(warmup):
+                apply_batch(items)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                apply_batch(items)
+            e1.record()
+            e1.synchronize()
+            log(f"[mrhs pmc] rhs_block {R}, {block} threads, t_block {t_block}: "
+                f"{e0.elapsed_time(e1) / reps * 1e3:.1f} us/launch")
+    finally:
+        set_launch(F32, 18, False, saved["rhs_block"], saved["block_threads"], saved["grid_blocks"],
+                   saved["t_block"])
+
+
 def mrhs_leg(reps: int = 50):
     """32 right-hand sides on one 24^3x48 gauge, and one persistent launch over
     8^4, 16^3x32, 24^3x48 with 4 RHS each (FP32, 18-real links), tuned shapes."""
@@ -412,6 +446,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 3D / 12-real variants")
     ap.add_argument("--no-cg", action="store_true", help="skip the 48^3x96 mixed vs FP64 CG")
     ap.add_argument("--no-mrhs", action="store_true", help="skip the batched Dslash legs")
+    ap.add_argument("--pmc-mrhs", action="store_true",
+                    help="only the 32-RHS launch in two fixed LDS-staged shapes (for rocprofv3 --pmc runs)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="only a 2 GiB calibration copy and the headline kernel (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -427,6 +463,9 @@ def main() -> int:
         return 2
 
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    if args.pmc_mrhs:
+        mrhs_pmc_workload()
+        return 0
     if args.pmc_workload:
         copy_leg(2.0, 3)
         gpu_leg(args.steps, args.warmup, "f32", 4, 18)

@@ -24,6 +24,14 @@ class Stager {
  public:
   static constexpr size_t CHUNK = (size_t)32 << 20;
 
+  Stager() = default;
+  Stager(const Stager&) = delete;
+  Stager& operator=(const Stager&) = delete;
+  // A caller that never frees the library still exits cleanly: the workers are
+  // stopped and joined (joinable std::threads would terminate the process).  No
+  // HIP call here, the runtime may already be gone at static destruction.
+  ~Stager() { stop_workers(); }
+
   void start() {
     if (pinned) return;
     if (hipHostMalloc((void**)&pinned, 2 * CHUNK, 0) != hipSuccess) {
@@ -42,13 +50,7 @@ class Stager {
   }
 
   void finish() {
-    {
-      std::lock_guard<std::mutex> lk(m);
-      stop = true;
-    }
-    cv.notify_all();
-    for (std::thread& t : workers) t.join();
-    workers.clear();
+    stop_workers();
     for (hipEvent_t& e : ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (pinned) (void)hipHostFree(pinned);
@@ -104,6 +106,16 @@ class Stager {
 
  private:
   bool usable() const { return pinned && ev[0] && ev[1]; }
+
+  void stop_workers() {
+    {
+      std::lock_guard<std::mutex> lk(m);
+      stop = true;
+    }
+    cv.notify_all();
+    for (std::thread& t : workers) t.join();
+    workers.clear();
+  }
 
   hipError_t reuse(int b) {  // wait until the DMA that last used buffer b is done
     if (!busy[b]) return hipSuccess;

@@ -1,7 +1,8 @@
 // CPU check of the drop-in API's staging pool (sse_wilson_dslash_amd/csrc/stager.h)
 // against tests/hip_stub: round trips of sizes below, at and above the
 // pipelining threshold (partial last chunks), repeated start / finish cycles
-// of one object (a restarted pool must not replay the previous job), built
+// of one object (a restarted pool must not replay the previous job), a pool
+// destroyed while started (workers stopped, not terminate()), built
 // with -fsanitize=thread or =address by tests/test_library_cpu.py.
 #include <stdio.h>
 #include <string.h>
@@ -32,6 +33,17 @@ int main() {
     }
     st.finish();
   }
+#ifndef __SANITIZE_ADDRESS__
+  // a started pool destroyed without finish() (a caller that never frees the
+  // library) must stop its workers instead of terminating the process; its
+  // staging buffer is left to the exiting process on purpose, so this part runs
+  // in the ThreadSanitizer build only (LeakSanitizer would report that buffer)
+  {
+    wd::Stager leaked;
+    leaked.start();
+    if (leaked.h2d(d.data(), h.data(), (size_t)70 << 20, nullptr) != hipSuccess) return 1;
+  }
+#endif
   puts("stager ok");
   return 0;
 }

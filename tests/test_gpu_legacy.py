@@ -180,6 +180,31 @@ def test_legacy_staged_transfers(oracle, libs, prec, nd, latt, order):
         free()
 
 
+def test_legacy_exit_without_free():
+    """A caller that initialises the library and exits without free_sse_su3dslash
+    (the reference allows it) ends with status 0: the staging workers are stopped
+    at exit, not left running (which would terminate the process)."""
+    import os
+    import subprocess
+    import sys
+    code = ("import numpy as np\n"
+            "from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host\n"
+            "from sse_wilson_dslash_amd.legacy import SSEDslash\n"
+            "lg = SSEDslash(32)\n"
+            "latt = (8, 8, 8, 8)\n"
+            "u = gen_gauge_host(latt, 4, seed=1, dtype=np.float32)\n"
+            "psi = gen_spinor_host(latt, 4, seed=1, dtype=np.float32)\n"
+            "lg.init_sse_su3dslash(latt)\n"
+            "lg.init_sse_su3dslash_3d(latt)\n"
+            "res = np.zeros_like(psi)\n"
+            "lg.sse_su3dslash_wilson(u, psi, res, 1, 0)\n"
+            "assert lg.last_error() == 0 and np.abs(res).max() > 0\n"
+            "print('applied', flush=True)\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and "applied" in r.stdout, (r.returncode, r.stdout, r.stderr[-2000:])
+
+
 def test_legacy_semantics(oracle, libs):
     """Aliasing psi == res, isign no-op, refcounting, resident-gauge paths."""
     lg = libs[32]

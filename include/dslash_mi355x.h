@@ -129,15 +129,20 @@ int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int
  * site per thread; 4 or 8 = a thread walks that many consecutive slices of the
  * slowest coordinate of the site order (t in 4D, z in 3D) and keeps the
  * projected source spinor of its forward hop for the backward hop two slices
- * later.  Any lattice extent is allowed (the last chunk is shorter).  A launch
- * that would have fewer than 131072 threads at this length (small lattices,
- * short t or z) uses the next shorter one, down to 1.  The x + a D psi forms
- * always run one site per thread.  dslash_set_launch leaves it unchanged;
- * dslash_tune_apply tunes it with the block shape. */
+ * later; 16 = LDS-staged box tiles (4D FP32 only): a block of block_threads
+ * (>= 128, >= Lx/2) holds whole x rows of a ty x tz box of (y, z), walks 8
+ * t-slices, and stages each slice's source spinors in LDS for the x hops and
+ * the y / z hops inside the box (links non-temporal).  Any lattice extent is
+ * allowed (the last chunk is shorter).  A launch that would have fewer than
+ * 131072 threads at this length (small lattices, short t or z) uses the next
+ * shorter one, down to 1; box tiles that cannot run fall back to 8.  The x + a
+ * D psi forms always run one site per thread.  dslash_set_launch leaves it
+ * unchanged; dslash_tune_apply tunes it with the block shape. */
 int dslash_set_launch_march(int nd, int prec, int gauge_reals, int march);
 int dslash_get_launch_march(int nd, int prec, int gauge_reals, int* march);
 /* The marching length res = D psi actually runs with on this lattice (the set
- * one, or shorter where it would leave fewer than 131072 threads). */
+ * one, or shorter where it would leave fewer than 131072 threads or box tiles
+ * cannot run). */
 int dslash_launch_march_for(const int latt[4], int nd, int prec, int gauge_reals, int* march);
 
 /* ---- many right-hand sides, many lattices ----------------------------- */

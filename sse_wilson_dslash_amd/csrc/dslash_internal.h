@@ -10,9 +10,10 @@ namespace wd {
 struct LaunchCfg {
   int block;  // threads per block
   int remap;  // XCD-contiguous block order
-  int march;  // slices of the slowest coordinate per thread in res = D psi (1, 4 or 8)
+  int march;  // res = D psi: slices of the slowest coordinate per thread (1, 4 or 8), or
+              // 16 = LDS-staged box tiles (4D FP32)
 };
-inline bool march_ok(int m) { return m == 1 || m == 4 || m == 8; }
+inline bool march_ok(int m) { return m == 1 || m == 4 || m == 8 || m == 16; }
 LaunchCfg launch_cfg(int nd, int prec, int G);
 void set_launch_cfg(int nd, int prec, int G, LaunchCfg c);
 

@@ -28,6 +28,7 @@
 #include "philox.h"
 #include "status.h"
 #include "dslash_internal.h"
+#include "mrhs.h"
 
 namespace wd {
 
@@ -57,8 +58,9 @@ void clear_error() {
 // link reals); set by dslash_set_launch / dslash_set_launch_march or by the
 // autotuner.  Defaults are the shapes measured fastest at 32^3 x 64 on MI355X
 // (XCD-contiguous order is worth ~25%; marching 6-16% for FP64, sloppy FP64 and
-// the 3D operator; for 4D FP32 it is within 1% at 32^3 x 64 and 3-5% slower at
-// 48^3 x 96, and it gains nothing for 16-bit spinors).
+// the 3D operator, nothing for 16-bit spinors; for 4D FP32, LDS-staged box tiles
+// (march 16) are 6-7% faster than one site per thread at 32^3 x 64 and within
+// 1% at 48^3 x 96, where marching is 3-5% slower).
 // Index: [nd == 4][F32, F64, F16, BF16, F64S][G == 12].
 static LaunchCfg g_cfg[2][NUM_PREC][2] = {
     {{{256, 1, 8}, {256, 1, 8}},
@@ -66,7 +68,7 @@ static LaunchCfg g_cfg[2][NUM_PREC][2] = {
      {{64, 1, 1}, {64, 1, 8}},
      {{64, 1, 1}, {64, 1, 1}},
      {{256, 1, 8}, {256, 1, 8}}},
-    {{{64, 1, 1}, {64, 1, 1}},
+    {{{256, 1, 16}, {256, 1, 16}},
      {{256, 1, 8}, {256, 1, 8}},
      {{256, 1, 1}, {256, 1, 1}},
      {{256, 1, 1}, {256, 1, 1}},
@@ -336,10 +338,18 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
 // sites per slice or few slices that would leave CUs idle, so a shorter march
 // (down to one site per thread) is used when the launch would fall below
 // MARCH_MIN_THREADS threads.
-static int effective_march(const int latt[4], int nd, int march) {
+// Box tiles (16) need the 4D FP32 operator, a whole x row per block and the
+// same thread floor; otherwise they fall back to a march of 8.
+static int effective_march(const int latt[4], int nd, int prec, int block, int march) {
   const uint64_t L = (uint64_t)latt[nd - 1];
   const uint64_t slice = (uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2 / L;
   int m = march_ok(march) ? march : 1;
+  if (m == 16) {
+    if (nd == 4 && prec == F32 && block >= 128 &&
+        (uint64_t)box_tiles_single(latt, block) * (uint64_t)block >= MARCH_MIN_THREADS)
+      return 16;
+    m = 8;
+  }
   while (m > 1 && slice * ((L + m - 1) / m) < MARCH_MIN_THREADS) m = m == 8 ? 4 : 1;
   return m;
 }
@@ -380,7 +390,12 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
   a.block = lc.block;
   a.slice = a.g.vh / (uint32_t)latt[nd - 1];
   a.dslice = make_fastdiv(a.slice);
-  a.march = effective_march(latt, nd, lc.march);
+  a.march = effective_march(latt, nd, prec, lc.block, lc.march);
+  if (a.march == 16 && !xp_cb) {  // LDS-staged box tiles
+    if (int e = box_apply_single(res_cb, psi_cb, gauge_cb, latt, G, isign, tcb, lc.block, s)) return e;
+    return 0;
+  }
+  if (a.march == 16) a.march = 1;  // the x + a D psi forms run one site per thread
   if (nd == 4)
     launch_dslash_nd<4>(a, prec, G, isign, s);
   else
@@ -612,7 +627,7 @@ int dslash_set_launch(int nd, int prec, int gauge_reals, int block_threads, int 
 
 int dslash_set_launch_march(int nd, int prec, int gauge_reals, int march) {
   clear_error();
-  WD_REQUIRE(march_ok(march), "march must be 1, 4 or 8 (got %d)", march);
+  WD_REQUIRE(march_ok(march), "march must be 1, 4, 8 or 16 (got %d)", march);
   WD_REQUIRE(nd == 3 || nd == 4, "nd must be 3 or 4");
   if (int e = check_prec(prec)) return e;
   WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
@@ -628,7 +643,8 @@ int dslash_launch_march_for(const int latt[4], int nd, int prec, int gauge_reals
   if (int e = check_prec(prec)) return e;
   WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");
   WD_REQUIRE(march, "march is NULL");
-  *march = effective_march(latt, nd, launch_cfg(nd, prec, gauge_reals).march);
+  const LaunchCfg c = launch_cfg(nd, prec, gauge_reals);
+  *march = effective_march(latt, nd, prec, c.block, c.march);
   return 0;
 }
 

@@ -73,5 +73,11 @@ inline bool mrhs_box_compatible(const dslash_mrhs_problem* probs, int nprob, int
   return true;
 }
 int mrhs_launch(const MrhsList& list, int prec, int G, int R, int block, int t_block, uint32_t grid, hipStream_t s);
+// res = D psi for one FP32 4D field pair with box tiles and non-temporal links
+// (the single-RHS path's march 16); needs Lx/2 <= block and block >= 128.
+int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int G, int isign,
+                     int tcb, int block, hipStream_t s);
+// Blocks (tiles) of that launch; 0 when Lx/2 > block.
+uint32_t box_tiles_single(const int latt[4], int block);
 
 }  // namespace wd

@@ -323,14 +323,14 @@ struct BoxLane {
   bool live;       // a site of the lattice (partial boxes at the y / z ends)
 };
 
-template <int G, int IS, int R, int MU>
+template <int G, int IS, int R, int MU, bool NTL>
 __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, const BoxLane& b, const Coord& c,
                                         uint32_t idx, uint32_t tS, int r0, Spinor<float> (&acc)[R]) {
   constexpr int PV = PairVec<F32, G>::value;
   const uint32_t vh = L.g.vh, nt = blockDim.x;
   const int lxh = L.g.lxh;
   C<float> Wf[3][3], Wb[3][3];
-  load_pair<F32, G, false>(L.gauge + (size_t)MU * PV * vh, idx, vh, Wf, Wb);
+  load_pair<F32, G, NTL>(L.gauge + (size_t)MU * PV * vh, idx, vh, Wf, Wb);
   const Coord cf = shift_coord(L.g, c, MU, +1), cb = shift_coord(L.g, c, MU, -1);
   // neighbour lanes inside the box (x: always; y / z: when the shifted
   // coordinate stays in the box's range, wrap included)
@@ -367,8 +367,10 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
 }
 
 // One box tile: box (bty, btz) of slices t0 .. t0+7 (clipped to T), fields
-// r0 .. r0+R-1.  Every thread executes the same barriers.
-template <int G, int IS, int R>
+// r0 .. r0+R-1.  Every thread executes the same barriers.  NTL: links loaded
+// non-temporally (one field: every link is used once), else cached for the
+// other RHS blocks of the same box.
+template <int G, int IS, int R, bool NTL>
 __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0, int r0) {
   constexpr int TB = MRHS_T_BLOCK_LDS;
   constexpr int PV = PairVec<F32, G>::value;
@@ -421,7 +423,7 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
          // (the forward neighbour), which is the next step's slab.  No other
          // lane reads that buffer during this step.
         C<float> Wf[3][3], Wb[3][3];
-        load_pair<F32, G, false>(L.gauge + (size_t)3 * PV * vh, idx, vh, Wf, Wb);
+        load_pair<F32, G, NTL>(L.gauge + (size_t)3 * PV * vh, idx, vh, Wf, Wb);
         const uint32_t nf = s + S * (uint32_t)(t + 1 == LT ? 0 : t + 1);
         v4u* other = slab + (size_t)(cur ^ 1) * bufsz;
 #pragma unroll
@@ -439,11 +441,11 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
       }
       const v4u* here = slab + (size_t)cur * bufsz;
       __builtin_amdgcn_sched_barrier(0);
-      box_hop<G, IS, R, 0>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<G, IS, R, 0, NTL>(L, here, b, c, idx, tS, r0, acc);
       __builtin_amdgcn_sched_barrier(0);
-      box_hop<G, IS, R, 1>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<G, IS, R, 1, NTL>(L, here, b, c, idx, tS, r0, acc);
       __builtin_amdgcn_sched_barrier(0);
-      box_hop<G, IS, R, 2>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<G, IS, R, 2, NTL>(L, here, b, c, idx, tS, r0, acc);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int r = 0; r < R; ++r) store_spinor<F32, true>(L.res + (size_t)(r0 + r) * L.fstride, nullptr, idx, vh, acc[r]);
@@ -453,7 +455,7 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
   __syncthreads();  // the next tile's preamble overwrites buffer 0
 }
 
-template <int G, int R, int IS>
+template <int G, int R, int IS, bool NTL>
 __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list) {
   extern __shared__ v4u box_slab[];
   const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
@@ -473,7 +475,7 @@ __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list) {
     const uint32_t q2 = q / L.nsb, box = q - q2 * L.nsb;
     const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
     const uint32_t btz = box / L.nty, bty = box - btz * L.nty;
-    mrhs_box_tile<G, IS, R>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS, (int)(grp * L.rg + rbi) * R);
+    mrhs_box_tile<G, IS, R, NTL>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS, (int)(grp * L.rg + rbi) * R);
   }
 }
 
@@ -529,11 +531,11 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
       const bool plus = list.lat[0].isign > 0;
       const size_t lds = (size_t)2 * R * 6 * block * sizeof(v4u);
       if (R == 1) {
-        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 1, 1>), dim3(grid), dim3(block), lds, s, list);
-        else hipLaunchKernelGGL((k_mrhs_box<G, 1, -1>), dim3(grid), dim3(block), lds, s, list);
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box<G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list);
       } else {
-        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 2, 1>), dim3(grid), dim3(block), lds, s, list);
-        else hipLaunchKernelGGL((k_mrhs_box<G, 2, -1>), dim3(grid), dim3(block), lds, s, list);
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 2, 1, false>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box<G, 2, -1, false>), dim3(grid), dim3(block), lds, s, list);
       }
     }
   } else if constexpr (P == F32) {  // MRHS_T_BLOCK_LDS (validated by mrhs_shape_ok; one isign per list)
@@ -633,6 +635,41 @@ int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R
   WD_REQUIRE(tiles < (1ull << 31), "too many tiles");
   list->n = nprob;
   list->total_tiles = (uint32_t)tiles;
+  return 0;
+}
+
+uint32_t box_tiles_single(const int latt[4], int block) {
+  const int lxh = latt[0] / 2;
+  if (lxh > block) return 0;
+  uint32_t ty, tz;
+  box_shape(lxh, latt[1], latt[2], block, &ty, &tz);
+  const uint32_t nbox = ((latt[1] + ty - 1) / ty) * ((latt[2] + tz - 1) / tz);
+  return nbox * (uint32_t)((latt[3] + MRHS_T_BLOCK_LDS - 1) / MRHS_T_BLOCK_LDS);
+}
+
+int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int G, int isign,
+                     int tcb, int block, hipStream_t s) {
+  dslash_mrhs_problem p;
+  for (int mu = 0; mu < 4; ++mu) p.latt[mu] = latt[mu];
+  p.d_gauge_cb = gauge_cb;
+  p.d_psi = psi_cb;
+  p.d_res = res_cb;
+  p.field_stride = spinor_cb_bytes(make_geom(latt, 4).vh, F32);
+  p.nrhs = 1;
+  p.isign = isign;
+  p.target_cb = tcb;
+  MrhsList list;
+  if (int e = mrhs_build_list(&p, 1, F32, 1, block, MRHS_T_BOX, &list)) return e;
+  const size_t lds = (size_t)2 * 6 * block * sizeof(v4u);
+  const dim3 grid(list.total_tiles), blk(block);
+  if (G == 18) {
+    if (isign > 0) hipLaunchKernelGGL((k_mrhs_box<18, 1, 1, true>), grid, blk, lds, s, list);
+    else hipLaunchKernelGGL((k_mrhs_box<18, 1, -1, true>), grid, blk, lds, s, list);
+  } else {
+    if (isign > 0) hipLaunchKernelGGL((k_mrhs_box<12, 1, 1, true>), grid, blk, lds, s, list);
+    else hipLaunchKernelGGL((k_mrhs_box<12, 1, -1, true>), grid, blk, lds, s, list);
+  }
+  WD_LAUNCHED();
   return 0;
 }
 

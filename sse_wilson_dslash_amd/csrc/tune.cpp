@@ -135,15 +135,21 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   // Candidates are timed in three interleaved rounds and scored by their best
   // round, so a transient slowdown of the device does not pick a shape.
   const int blocks[4] = {64, 128, 192, 256};
-  const int marches[3] = {1, 4, 8};
-  float tmin[4][2][3];
+  const int marches[4] = {1, 4, 8, 16};
+  const int nm = (nd == 4 && prec == F32) ? 4 : 3;  // box tiles (16): 4D FP32 only
+  float tmin[4][2][4];
   for (auto& a : tmin)
     for (auto& b : a)
       for (float& t : b) t = 1e30f;
   for (int round = 0; round < 3; ++round)
     for (int bi = 0; bi < 4; ++bi)
       for (int remap : {0, 1})
-        for (int mi = 0; mi < 3; ++mi) {
+        for (int mi = 0; mi < nm; ++mi) {
+          // box tiles ignore the remap flag (their tile order is XCD-contiguous by
+          // construction); tuning them with remap 1 only keeps the flag right for
+          // the one-site kernels that share this entry (x + a D psi forms, the
+          // domain-decomposition interior, lattices below the box thread floor)
+          if (marches[mi] == 16 && (blocks[bi] < 128 || remap == 0)) continue;
           set_launch_cfg(nd, prec, G, LaunchCfg{blocks[bi], remap, marches[mi]});
           float t = 0;
           int k = 0;
@@ -162,7 +168,7 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   float best_t = 1e30f;
   for (int bi = 0; bi < 4; ++bi)
     for (int remap : {0, 1})
-      for (int mi = 0; mi < 3; ++mi)
+      for (int mi = 0; mi < nm; ++mi)
         if (tmin[bi][remap][mi] < best_t) {
           best_t = tmin[bi][remap][mi];
           best = LaunchCfg{blocks[bi], remap, marches[mi]};

@@ -256,6 +256,7 @@ def test_marching_falls_back_on_small_lattices(oracle, nd, latt, prec):
 # z = 34 leave a 2-slice last chunk at march 4 and 8; t = 2 and z = 2 are shorter
 # than the march; t = 65 is an odd time extent of the 3D operator.
 MARCH_CASES = [(4, (32, 32, 32, 66), F64, 18), (4, (32, 32, 32, 66), F32, 12), (4, (32, 32, 32, 66), BF16, 12),
+               (4, (24, 48, 48, 48), F32, 18),  # box tiles of 12 x 4 x 4 sites: 64 idle lanes per block
                (4, (64, 64, 64, 2), F64S, 18), (4, (64, 64, 64, 2), F16, 12),
                (3, (32, 32, 34, 65), F64, 12), (3, (32, 32, 34, 65), F32, 18), (3, (32, 32, 34, 65), F16, 18),
                (3, (64, 64, 2, 64), BF16, 18), (3, (64, 64, 2, 64), F64S, 12)]
@@ -264,9 +265,10 @@ MARCH_CASES = [(4, (32, 32, 32, 66), F64, 18), (4, (32, 32, 32, 66), F32, 12), (
 @pytest.mark.parametrize("nd,latt,prec,G", MARCH_CASES)
 def test_marching_lengths(oracle, nd, latt, prec, G):
     """res = D psi with a thread walking 1, 4 or 8 slices of the slowest
-    coordinate (t in 4D, z in 3D): every march agrees with the oracle and with
-    the one-site-per-thread result, and leaves the source checkerboard of res
-    untouched."""
+    coordinate (t in 4D, z in 3D), and for the 4D FP32 operator with LDS-staged
+    box tiles (march 16, 256 threads): every shape agrees with the oracle and
+    with the one-site-per-thread result, and leaves the source checkerboard of
+    res untouched."""
     print(f"[march] nd={nd} latt={latt} prec={prec} G={G}", flush=True)
     u = gen_gauge_host(latt, nd, seed=3)
     psi = gen_spinor_host(latt, nd, seed=3, field=2)
@@ -277,14 +279,21 @@ def test_marching_lengths(oracle, nd, latt, prec, G):
     op = WilsonDslash(latt, nd=nd, prec=prec, gauge_reals=G)
     op.load_gauge(u)
     d_psi = op.pack_spinor(psi)
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    lib = _lib.core()
+    b0, r0 = C.c_int(), C.c_int()
+    _lib.check(lib.dslash_get_launch(nd, prec, G, C.byref(b0), C.byref(r0)))
+    marches = (1, 4, 8, 16) if (nd == 4 and prec == F32) else (1, 4, 8)
     with _March(nd, prec, G) as m:
         for isign, cb in ((1, 0), (-1, 1)):
             want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
             t = slice((1 - cb) * vh, (2 - cb) * vh)
             w = want.reshape(-1, 24)[t]
             first = None
-            for march in (1, 4, 8):
+            for march in marches:
                 m.set(march)
+                _lib.check(lib.dslash_set_launch(nd, prec, G, 256 if march == 16 else b0.value, r0.value))
                 assert _march_for(latt, nd, prec, G) == march
                 d_res = op.pack_spinor(np.full_like(psi, 3.25))
                 op.apply(d_res, d_psi, isign, cb)
@@ -301,6 +310,7 @@ def test_marching_lengths(oracle, nd, latt, prec, G):
                 else:
                     assert np.max(np.abs(g - first)) <= MARCH_SAME[prec] * np.max(np.abs(first)), \
                         (march, isign, cb)
+    _lib.check(lib.dslash_set_launch(nd, prec, G, b0.value, r0.value))
     del op, d_psi
     torch.cuda.empty_cache()
 

@@ -162,8 +162,8 @@ def test_march_setting():
     m = C.c_int()
     assert lib.dslash_get_launch_march(3, _lib.F32, 18, C.byref(m)) == 0
     saved = m.value
-    assert saved in (1, 4, 8)
-    for bad in (0, 2, 3, 16, -1):
+    assert saved in (1, 4, 8, 16)
+    for bad in (0, 2, 3, 32, -1):
         assert lib.dslash_set_launch_march(3, _lib.F32, 18, bad) == _lib.ERR_INVALID
     assert b"march" in lib.dslash_last_error_message()
     assert lib.dslash_set_launch_march(5, _lib.F32, 18, 4) == _lib.ERR_INVALID
@@ -204,11 +204,43 @@ def test_march_falls_back_below_the_thread_floor():
         assert eff((16, 16, 16, 64), 3) == 1
         assert lib.dslash_set_launch_march(4, _lib.F64, 18, 1) == 0
         assert eff((32, 32, 32, 64), 4) == 1
+        # box tiles (16) exist for the 4D FP32 operator only; elsewhere they fall back to 8
+        assert lib.dslash_set_launch_march(4, _lib.F64, 18, 16) == 0
+        assert eff((32, 32, 32, 64), 4) == 8
+        assert lib.dslash_set_launch_march(3, _lib.F64, 18, 16) == 0
+        assert eff((32, 32, 32, 64), 3) == 8
         assert lib.dslash_launch_march_for(_lib.latt_arg((3, 4, 4, 4)), 4, _lib.F64, 18, C.byref(m)) == \
             _lib.ERR_INVALID
     finally:
         lib.dslash_set_launch_march(4, _lib.F64, 18, saved4)
         lib.dslash_set_launch_march(3, _lib.F64, 18, saved3)
+
+
+def test_box_tiles_single_rhs_fallbacks():
+    """march 16 (box tiles) runs where the 4D FP32 launch has >= 128 threads per
+    block, a whole x row per block and >= 131072 threads; otherwise march 8 (then
+    the thread floor applies as for any march)."""
+    lib = _lib.core()
+    b, r, m = C.c_int(), C.c_int(), C.c_int()
+    lib.dslash_get_launch(4, _lib.F32, 18, C.byref(b), C.byref(r))
+    lib.dslash_get_launch_march(4, _lib.F32, 18, C.byref(m))
+    saved = (b.value, r.value, m.value)
+
+    def eff(latt):
+        assert lib.dslash_launch_march_for(_lib.latt_arg(latt), 4, _lib.F32, 18, C.byref(m)) == 0
+        return m.value
+    try:
+        assert lib.dslash_set_launch_march(4, _lib.F32, 18, 16) == 0
+        assert lib.dslash_set_launch(4, _lib.F32, 18, 256, 1) == 0
+        assert eff((32, 32, 32, 64)) == 16   # 64 boxes of 16 x 4 x 4 x 8 chunks = 512 blocks of 256
+        assert eff((48, 48, 48, 96)) == 16
+        assert eff((8, 8, 8, 8)) == 1        # one box: below the thread floor, and so is march 8
+        assert eff((600, 8, 8, 64)) == 8     # Lx/2 = 300 > 256 threads
+        assert lib.dslash_set_launch(4, _lib.F32, 18, 64, 1) == 0
+        assert eff((32, 32, 32, 64)) == 8    # 64 threads per block: no box tiles
+    finally:
+        lib.dslash_set_launch(4, _lib.F32, 18, saved[0], saved[1])
+        lib.dslash_set_launch_march(4, _lib.F32, 18, saved[2])
 
 
 def test_tune_file_round_trip(tmp_path):

@@ -60,7 +60,8 @@ void clear_error() {
 // (XCD-contiguous order is worth ~25%; marching 6-16% for FP64, sloppy FP64 and
 // the 3D operator, nothing for 16-bit spinors; for 4D FP32, LDS-staged box tiles
 // (march 16) are 6-7% faster than one site per thread at 32^3 x 64 and within
-// 1% at 48^3 x 96, where marching is 3-5% slower).
+// 1% at 48^3 x 96, where marching is 3-5% slower; for 4D FP64 they are 3-4%
+// faster than a march of 8).
 // Index: [nd == 4][F32, F64, F16, BF16, F64S][G == 12].
 static LaunchCfg g_cfg[2][NUM_PREC][2] = {
     {{{256, 1, 8}, {256, 1, 8}},
@@ -69,7 +70,7 @@ static LaunchCfg g_cfg[2][NUM_PREC][2] = {
      {{64, 1, 1}, {64, 1, 1}},
      {{256, 1, 8}, {256, 1, 8}}},
     {{{256, 1, 16}, {256, 1, 16}},
-     {{256, 1, 8}, {256, 1, 8}},
+     {{256, 1, 16}, {256, 1, 16}},
      {{256, 1, 1}, {256, 1, 1}},
      {{256, 1, 1}, {256, 1, 1}},
      {{256, 1, 8}, {256, 1, 8}}}};
@@ -338,14 +339,14 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
 // sites per slice or few slices that would leave CUs idle, so a shorter march
 // (down to one site per thread) is used when the launch would fall below
 // MARCH_MIN_THREADS threads.
-// Box tiles (16) need the 4D FP32 operator, a whole x row per block and the
-// same thread floor; otherwise they fall back to a march of 8.
+// Box tiles (16) need the 4D FP32 or FP64 operator, a whole x row per block and
+// the same thread floor; otherwise they fall back to a march of 8.
 static int effective_march(const int latt[4], int nd, int prec, int block, int march) {
   const uint64_t L = (uint64_t)latt[nd - 1];
   const uint64_t slice = (uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2 / L;
   int m = march_ok(march) ? march : 1;
   if (m == 16) {
-    if (nd == 4 && prec == F32 && block >= 128 &&
+    if (nd == 4 && (prec == F32 || prec == F64) && block >= 128 &&
         (uint64_t)box_tiles_single(latt, block) * (uint64_t)block >= MARCH_MIN_THREADS)
       return 16;
     m = 8;
@@ -392,7 +393,7 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
   a.dslice = make_fastdiv(a.slice);
   a.march = effective_march(latt, nd, prec, lc.block, lc.march);
   if (a.march == 16 && !xp_cb) {  // LDS-staged box tiles
-    if (int e = box_apply_single(res_cb, psi_cb, gauge_cb, latt, G, isign, tcb, lc.block, s)) return e;
+    if (int e = box_apply_single(res_cb, psi_cb, gauge_cb, latt, prec, G, isign, tcb, lc.block, s)) return e;
     return 0;
   }
   if (a.march == 16) a.march = 1;  // the x + a D psi forms run one site per thread

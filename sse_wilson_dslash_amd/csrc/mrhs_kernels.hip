@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(MRHS_LDS_THREADS) k_mrhs_lds(MrhsList list) {
 // L2 (4 MB per XCD) turns over faster than the neighbour blocks reach the same
 // slice, so those loads were re-fetched from past L2 (3.7 GB per launch at
 // 32 RHS on 24^3 x 48 against 1.2 GB compulsory reads).
-// Slab layout (dynamic LDS): [2 buffers][R][6 vectors][blockDim.x lanes].
+// Slab layout (dynamic LDS): [2 buffers][R][6 (FP32) or 12 (FP64) vectors][blockDim.x lanes].
 // ---------------------------------------------------------------------------
 struct BoxLane {
   int i;           // x row position (= x / 2)
@@ -323,14 +323,33 @@ struct BoxLane {
   bool live;       // a site of the lattice (partial boxes at the y / z ends)
 };
 
-template <int G, int IS, int R, int MU, bool NTL>
+// One FP32 / FP64 spinor from its 16-byte vectors at p, p + stride, ...
+template <int P> __device__ inline Spinor<typename Fmt<P>::compute> box_spinor(const v4u* p, size_t stride) {
+  typedef typename Fmt<P>::real Rl;
+  constexpr int NV = 24 * Fmt<P>::bytes / 16;
+  v4u buf[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) buf[v] = p[(size_t)v * stride];
+  const Rl* r = reinterpret_cast<const Rl*>(buf);
+  Spinor<typename Fmt<P>::compute> q;
+#pragma unroll
+  for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) q.s[sp][cc] = {r[(sp * 3 + cc) * 2], r[(sp * 3 + cc) * 2 + 1]};
+  return q;
+}
+
+template <int P, int G, int IS, int R, int MU, bool NTL>
 __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, const BoxLane& b, const Coord& c,
-                                        uint32_t idx, uint32_t tS, int r0, Spinor<float> (&acc)[R]) {
-  constexpr int PV = PairVec<F32, G>::value;
+                                        uint32_t idx, uint32_t tS, int r0,
+                                        Spinor<typename Fmt<P>::compute> (&acc)[R]) {
+  typedef typename Fmt<P>::compute T;
+  constexpr int PV = PairVec<P, G>::value;
+  constexpr int NV = 24 * Fmt<P>::bytes / 16;
   const uint32_t vh = L.g.vh, nt = blockDim.x;
   const int lxh = L.g.lxh;
-  C<float> Wf[3][3], Wb[3][3];
-  load_pair<F32, G, NTL>(L.gauge + (size_t)MU * PV * vh, idx, vh, Wf, Wb);
+  C<T> Wf[3][3], Wb[3][3];
+  load_pair<P, G, NTL>(L.gauge + (size_t)MU * PV * vh, idx, vh, Wf, Wb);
   const Coord cf = shift_coord(L.g, c, MU, +1), cb = shift_coord(L.g, c, MU, -1);
   // neighbour lanes inside the box (x: always; y / z: when the shifted
   // coordinate stays in the box's range, wrap included)
@@ -356,11 +375,11 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + tS;
-    const v4u* sr = slab + (size_t)r * 6 * nt;
+    const v4u* sr = slab + (size_t)r * NV * nt;
     const v4u* pf = inf ? sr + lf : f + nf;
     const v4u* pb = inb ? sr + lb : f + nb;
-    const Spinor<float> p = spinor_at(pf, inf ? (size_t)nt : (size_t)vh);
-    const Spinor<float> q = spinor_at(pb, inb ? (size_t)nt : (size_t)vh);
+    const Spinor<T> p = box_spinor<P>(pf, inf ? (size_t)nt : (size_t)vh);
+    const Spinor<T> q = box_spinor<P>(pb, inb ? (size_t)nt : (size_t)vh);
     spin_recon_add<MU, -IS>(acc[r], su3_mul(Wf, spin_project<MU, -IS>(p)));
     spin_recon_add<MU, IS>(acc[r], su3_mul(Wb, spin_project<MU, IS>(q)));
   }
@@ -370,10 +389,12 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
 // r0 .. r0+R-1.  Every thread executes the same barriers.  NTL: links loaded
 // non-temporally (one field: every link is used once), else cached for the
 // other RHS blocks of the same box.
-template <int G, int IS, int R, bool NTL>
+template <int P, int G, int IS, int R, bool NTL>
 __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0, int r0) {
+  typedef typename Fmt<P>::compute T;
   constexpr int TB = MRHS_T_BLOCK_LDS;
-  constexpr int PV = PairVec<F32, G>::value;
+  constexpr int PV = PairVec<P, G>::value;
+  constexpr int NV = 24 * Fmt<P>::bytes / 16;
   const uint32_t vh = L.g.vh, S = L.S, nt = blockDim.x;
   const int LT = L.g.L[3], lxh = L.g.lxh;
   BoxLane b;
@@ -388,16 +409,16 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
     b.live = k < (int)L.tz && b.y < L.g.L[1] && b.z < L.g.L[2];
   }
   const uint32_t s = (uint32_t)b.i + (uint32_t)lxh * ((uint32_t)b.y + (uint32_t)L.g.L[1] * (uint32_t)b.z);
-  const size_t bufsz = (size_t)R * 6 * nt;
+  const size_t bufsz = (size_t)R * NV * nt;
   if (b.live) {  // preamble: slice t0 at the own site -> buffer 0, slice t0 - 1 -> buffer 1
     const uint32_t tm = (uint32_t)(t0 == 0 ? LT - 1 : t0 - 1);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride + s;
 #pragma unroll
-      for (int v = 0; v < 6; ++v) {
-        slab[((size_t)r * 6 + v) * nt + threadIdx.x] = f[(size_t)v * vh + S * (uint32_t)t0];
-        slab[bufsz + ((size_t)r * 6 + v) * nt + threadIdx.x] = f[(size_t)v * vh + S * tm];
+      for (int v = 0; v < NV; ++v) {
+        slab[((size_t)r * NV + v) * nt + threadIdx.x] = f[(size_t)v * vh + S * (uint32_t)t0];
+        slab[bufsz + ((size_t)r * NV + v) * nt + threadIdx.x] = f[(size_t)v * vh + S * tm];
       }
     }
   }
@@ -411,51 +432,51 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
       const Coord c{2 * b.i + ((L.tcb + b.y + b.z + t) & 1), b.y, b.z, t};
       const uint32_t tS = S * (uint32_t)t;
       const uint32_t idx = s + tS;
-      Spinor<float> acc[R];
+      Spinor<T> acc[R];
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int sp = 0; sp < 4; ++sp)
 #pragma unroll
-          for (int cc = 0; cc < 3; ++cc) acc[r].s[sp][cc] = {0.f, 0.f};
+          for (int cc = 0; cc < 3; ++cc) acc[r].s[sp][cc] = {(T)0, (T)0};
       {  // t hops.  The own slot of the other buffer holds slice t - 1 (the
          // backward neighbour); it is read, then overwritten with slice t + 1
          // (the forward neighbour), which is the next step's slab.  No other
          // lane reads that buffer during this step.
-        C<float> Wf[3][3], Wb[3][3];
-        load_pair<F32, G, NTL>(L.gauge + (size_t)3 * PV * vh, idx, vh, Wf, Wb);
+        C<T> Wf[3][3], Wb[3][3];
+        load_pair<P, G, NTL>(L.gauge + (size_t)3 * PV * vh, idx, vh, Wf, Wb);
         const uint32_t nf = s + S * (uint32_t)(t + 1 == LT ? 0 : t + 1);
         v4u* other = slab + (size_t)(cur ^ 1) * bufsz;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const v4u* f = L.psi + (size_t)(r0 + r) * L.fstride;
-          const Spinor<float> q = spinor_at(other + (size_t)r * 6 * nt + threadIdx.x, nt);
+          const Spinor<T> q = box_spinor<P>(other + (size_t)r * NV * nt + threadIdx.x, nt);
           spin_recon_add<3, IS>(acc[r], su3_mul(Wb, spin_project<3, IS>(q)));
-          v4u buf[6];
+          v4u buf[NV];
 #pragma unroll
-          for (int v = 0; v < 6; ++v) buf[v] = f[(size_t)v * vh + nf];
+          for (int v = 0; v < NV; ++v) buf[v] = f[(size_t)v * vh + nf];
 #pragma unroll
-          for (int v = 0; v < 6; ++v) other[((size_t)r * 6 + v) * nt + threadIdx.x] = buf[v];
-          spin_recon_add<3, -IS>(acc[r], su3_mul(Wf, spin_project<3, -IS>(spinor_of(buf))));
+          for (int v = 0; v < NV; ++v) other[((size_t)r * NV + v) * nt + threadIdx.x] = buf[v];
+          spin_recon_add<3, -IS>(acc[r], su3_mul(Wf, spin_project<3, -IS>(box_spinor<P>(buf, 1))));
         }
       }
       const v4u* here = slab + (size_t)cur * bufsz;
       __builtin_amdgcn_sched_barrier(0);
-      box_hop<G, IS, R, 0, NTL>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<P, G, IS, R, 0, NTL>(L, here, b, c, idx, tS, r0, acc);
       __builtin_amdgcn_sched_barrier(0);
-      box_hop<G, IS, R, 1, NTL>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<P, G, IS, R, 1, NTL>(L, here, b, c, idx, tS, r0, acc);
       __builtin_amdgcn_sched_barrier(0);
-      box_hop<G, IS, R, 2, NTL>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<P, G, IS, R, 2, NTL>(L, here, b, c, idx, tS, r0, acc);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) store_spinor<F32, true>(L.res + (size_t)(r0 + r) * L.fstride, nullptr, idx, vh, acc[r]);
+      for (int r = 0; r < R; ++r) store_spinor<P, true>(L.res + (size_t)(r0 + r) * L.fstride, nullptr, idx, vh, acc[r]);
     }
     __syncthreads();  // buffer cur ^ 1 complete; buffer cur free for the next step's writes
   }
   __syncthreads();  // the next tile's preamble overwrites buffer 0
 }
 
-template <int G, int R, int IS, bool NTL>
+template <int P, int G, int R, int IS, bool NTL>
 __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list) {
   extern __shared__ v4u box_slab[];
   const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
@@ -475,7 +496,7 @@ __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list) {
     const uint32_t q2 = q / L.nsb, box = q - q2 * L.nsb;
     const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
     const uint32_t btz = box / L.nty, bty = box - btz * L.nty;
-    mrhs_box_tile<G, IS, R, NTL>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS, (int)(grp * L.rg + rbi) * R);
+    mrhs_box_tile<P, G, IS, R, NTL>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS, (int)(grp * L.rg + rbi) * R);
   }
 }
 
@@ -526,16 +547,19 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
   } else if (TB == MRHS_T_BLOCK) {
     if (R == 1) hipLaunchKernelGGL((k_mrhs<P, G, 1, MRHS_T_BLOCK>), dim3(grid), dim3(block), 0, s, list);
     else hipLaunchKernelGGL((k_mrhs<P, G, 2, MRHS_T_BLOCK>), dim3(grid), dim3(block), 0, s, list);
-  } else if (TB == MRHS_T_BOX) {
-    if constexpr (P == F32) {  // validated by mrhs_shape_ok / mrhs_box_compatible; one isign per list
-      const bool plus = list.lat[0].isign > 0;
-      const size_t lds = (size_t)2 * R * 6 * block * sizeof(v4u);
+  } else if (TB == MRHS_T_BOX) {  // validated by mrhs_shape_ok / mrhs_box_compatible; one isign per list
+    const bool plus = list.lat[0].isign > 0;
+    const size_t lds = (size_t)2 * R * spinor_vec(P) * block * sizeof(v4u);
+    if constexpr (P == F64) {  // FP64: one field per thread
+      if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list);
+      else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list);
+    } else {
       if (R == 1) {
-        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list);
-        else hipLaunchKernelGGL((k_mrhs_box<G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list);
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list);
       } else {
-        if (plus) hipLaunchKernelGGL((k_mrhs_box<G, 2, 1, false>), dim3(grid), dim3(block), lds, s, list);
-        else hipLaunchKernelGGL((k_mrhs_box<G, 2, -1, false>), dim3(grid), dim3(block), lds, s, list);
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 2, 1, false>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box<P, G, 2, -1, false>), dim3(grid), dim3(block), lds, s, list);
       }
     }
   } else if constexpr (P == F32) {  // MRHS_T_BLOCK_LDS (validated by mrhs_shape_ok; one isign per list)
@@ -647,27 +671,36 @@ uint32_t box_tiles_single(const int latt[4], int block) {
   return nbox * (uint32_t)((latt[3] + MRHS_T_BLOCK_LDS - 1) / MRHS_T_BLOCK_LDS);
 }
 
-int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int G, int isign,
-                     int tcb, int block, hipStream_t s) {
+template <int P, int G>
+static void launch_box_single(const MrhsList& list, int isign, int block, hipStream_t s) {
+  const size_t lds = (size_t)2 * spinor_vec(P) * block * sizeof(v4u);
+  const dim3 grid(list.total_tiles), blk(block);
+  if (isign > 0)
+    hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, true>), grid, blk, lds, s, list);
+  else
+    hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, true>), grid, blk, lds, s, list);
+}
+
+int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
+                     int isign, int tcb, int block, hipStream_t s) {
+  WD_REQUIRE(prec == F32 || prec == F64, "box tiles are built for F32 and F64");
   dslash_mrhs_problem p;
   for (int mu = 0; mu < 4; ++mu) p.latt[mu] = latt[mu];
   p.d_gauge_cb = gauge_cb;
   p.d_psi = psi_cb;
   p.d_res = res_cb;
-  p.field_stride = spinor_cb_bytes(make_geom(latt, 4).vh, F32);
+  p.field_stride = spinor_cb_bytes(make_geom(latt, 4).vh, prec);
   p.nrhs = 1;
   p.isign = isign;
   p.target_cb = tcb;
   MrhsList list;
-  if (int e = mrhs_build_list(&p, 1, F32, 1, block, MRHS_T_BOX, &list)) return e;
-  const size_t lds = (size_t)2 * 6 * block * sizeof(v4u);
-  const dim3 grid(list.total_tiles), blk(block);
-  if (G == 18) {
-    if (isign > 0) hipLaunchKernelGGL((k_mrhs_box<18, 1, 1, true>), grid, blk, lds, s, list);
-    else hipLaunchKernelGGL((k_mrhs_box<18, 1, -1, true>), grid, blk, lds, s, list);
+  if (int e = mrhs_build_list(&p, 1, prec, 1, block, MRHS_T_BOX, &list)) return e;
+  if (prec == F32) {
+    if (G == 18) launch_box_single<F32, 18>(list, isign, block, s);
+    else launch_box_single<F32, 12>(list, isign, block, s);
   } else {
-    if (isign > 0) hipLaunchKernelGGL((k_mrhs_box<12, 1, 1, true>), grid, blk, lds, s, list);
-    else hipLaunchKernelGGL((k_mrhs_box<12, 1, -1, true>), grid, blk, lds, s, list);
+    if (G == 18) launch_box_single<F64, 18>(list, isign, block, s);
+    else launch_box_single<F64, 12>(list, isign, block, s);
   }
   WD_LAUNCHED();
   return 0;

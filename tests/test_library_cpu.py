@@ -204,9 +204,15 @@ def test_march_falls_back_below_the_thread_floor():
         assert eff((16, 16, 16, 64), 3) == 1
         assert lib.dslash_set_launch_march(4, _lib.F64, 18, 1) == 0
         assert eff((32, 32, 32, 64), 4) == 1
-        # box tiles (16) exist for the 4D FP32 operator only; elsewhere they fall back to 8
+        # box tiles (16) exist for the 4D FP32 / FP64 operators; elsewhere they fall back to 8
+        b, r = C.c_int(), C.c_int()
+        lib.dslash_get_launch(4, _lib.F64, 18, C.byref(b), C.byref(r))
         assert lib.dslash_set_launch_march(4, _lib.F64, 18, 16) == 0
+        assert lib.dslash_set_launch(4, _lib.F64, 18, 256, 1) == 0
+        assert eff((32, 32, 32, 64), 4) == 16
+        assert lib.dslash_set_launch(4, _lib.F64, 18, 64, 1) == 0
         assert eff((32, 32, 32, 64), 4) == 8
+        assert lib.dslash_set_launch(4, _lib.F64, 18, b.value, r.value) == 0
         assert lib.dslash_set_launch_march(3, _lib.F64, 18, 16) == 0
         assert eff((32, 32, 32, 64), 3) == 8
         assert lib.dslash_launch_march_for(_lib.latt_arg((3, 4, 4, 4)), 4, _lib.F64, 18, C.byref(m)) == \

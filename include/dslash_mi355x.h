@@ -129,7 +129,7 @@ int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int
  * site per thread; 4 or 8 = a thread walks that many consecutive slices of the
  * slowest coordinate of the site order (t in 4D, z in 3D) and keeps the
  * projected source spinor of its forward hop for the backward hop two slices
- * later; 16 = LDS-staged box tiles (4D FP32 and FP64): a block of block_threads
+ * later; 16 = LDS-staged box tiles (4D FP32, FP64, sloppy FP64): a block of block_threads
  * (>= 128, >= Lx/2) holds whole x rows of a ty x tz box of (y, z), walks 8
  * t-slices, and stages each slice's source spinors in LDS for the x hops and
  * the y / z hops inside the box (links non-temporal).  Any lattice extent is

@@ -19,13 +19,14 @@ constexpr int MRHS_T_BOX = 16;       // t_block value of LDS-staged box tiles (8
 // Tile shapes: t_block 1 (one site per thread, any rhs_block), MRHS_T_BLOCK
 // (t-marching, rhs_block <= 2), MRHS_T_BLOCK_LDS (t-marching with an LDS slab,
 // F32, rhs_block <= 2, MRHS_LDS_THREADS threads), MRHS_T_BOX (LDS-staged box
-// tiles, F32 with rhs_block <= 2 or F64 with rhs_block 1, >= 128 threads).
+// tiles, F32 with rhs_block <= 2 or F64 with rhs_block 1, >= 128 threads; the
+// single-field path also takes F64S).
 inline bool mrhs_shape_ok(int prec, int R, int block, int t_block) {
   if (!(R == 1 || R == 2 || R == 4) || block < 64 || block > 256 || block % 64) return false;
   if (t_block == 1) return true;
   if (t_block == MRHS_T_BLOCK) return R <= 2;
   if (t_block == MRHS_T_BLOCK_LDS) return prec == F32 && R <= 2 && block == MRHS_LDS_THREADS;
-  if (t_block == MRHS_T_BOX) return (prec == F32 ? R <= 2 : prec == F64 && R == 1) && block >= 128;
+  if (t_block == MRHS_T_BOX) return (prec == F32 ? R <= 2 : (prec == F64 || prec == F64S) && R == 1) && block >= 128;
   return false;
 }
 
@@ -74,7 +75,7 @@ inline bool mrhs_box_compatible(const dslash_mrhs_problem* probs, int nprob, int
   return true;
 }
 int mrhs_launch(const MrhsList& list, int prec, int G, int R, int block, int t_block, uint32_t grid, hipStream_t s);
-// res = D psi for one FP32 or FP64 4D field pair with box tiles and
+// res = D psi for one FP32, FP64 or sloppy-FP64 4D field pair with box tiles and
 // non-temporal links (the single-RHS path's march 16); needs Lx/2 <= block and
 // block >= 128.
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,

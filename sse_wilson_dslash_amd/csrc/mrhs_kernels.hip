@@ -323,7 +323,8 @@ struct BoxLane {
   bool live;       // a site of the lattice (partial boxes at the y / z ends)
 };
 
-// One FP32 / FP64 spinor from its 16-byte vectors at p, p + stride, ...
+// One FP32 / FP64 (or sloppy FP64, converted to FP32) spinor from its 16-byte
+// vectors at p, p + stride, ...
 template <int P> __device__ inline Spinor<typename Fmt<P>::compute> box_spinor(const v4u* p, size_t stride) {
   typedef typename Fmt<P>::real Rl;
   constexpr int NV = 24 * Fmt<P>::bytes / 16;
@@ -335,7 +336,8 @@ template <int P> __device__ inline Spinor<typename Fmt<P>::compute> box_spinor(c
 #pragma unroll
   for (int sp = 0; sp < 4; ++sp)
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc) q.s[sp][cc] = {r[(sp * 3 + cc) * 2], r[(sp * 3 + cc) * 2 + 1]};
+    for (int cc = 0; cc < 3; ++cc)
+      q.s[sp][cc] = {to_compute<P>(r[(sp * 3 + cc) * 2]), to_compute<P>(r[(sp * 3 + cc) * 2 + 1])};
   return q;
 }
 
@@ -683,7 +685,7 @@ static void launch_box_single(const MrhsList& list, int isign, int block, hipStr
 
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
                      int isign, int tcb, int block, hipStream_t s) {
-  WD_REQUIRE(prec == F32 || prec == F64, "box tiles are built for F32 and F64");
+  WD_REQUIRE(prec == F32 || prec == F64 || prec == F64S, "box tiles are built for F32, F64 and F64S");
   dslash_mrhs_problem p;
   for (int mu = 0; mu < 4; ++mu) p.latt[mu] = latt[mu];
   p.d_gauge_cb = gauge_cb;
@@ -698,9 +700,12 @@ int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, con
   if (prec == F32) {
     if (G == 18) launch_box_single<F32, 18>(list, isign, block, s);
     else launch_box_single<F32, 12>(list, isign, block, s);
-  } else {
+  } else if (prec == F64) {
     if (G == 18) launch_box_single<F64, 18>(list, isign, block, s);
     else launch_box_single<F64, 12>(list, isign, block, s);
+  } else {
+    if (G == 18) launch_box_single<F64S, 18>(list, isign, block, s);
+    else launch_box_single<F64S, 12>(list, isign, block, s);
   }
   WD_LAUNCHED();
   return 0;

@@ -136,7 +136,7 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   // round, so a transient slowdown of the device does not pick a shape.
   const int blocks[4] = {64, 128, 192, 256};
   const int marches[4] = {1, 4, 8, 16};
-  const int nm = (nd == 4 && (prec == F32 || prec == F64)) ? 4 : 3;  // box tiles (16): 4D FP32 / FP64
+  const int nm = (nd == 4 && (prec == F32 || prec == F64 || prec == F64S)) ? 4 : 3;  // box tiles (16): 4D
   float tmin[4][2][4];
   for (auto& a : tmin)
     for (auto& b : a)

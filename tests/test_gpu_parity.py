@@ -265,8 +265,8 @@ MARCH_CASES = [(4, (32, 32, 32, 66), F64, 18), (4, (32, 32, 32, 66), F32, 12), (
 @pytest.mark.parametrize("nd,latt,prec,G", MARCH_CASES)
 def test_marching_lengths(oracle, nd, latt, prec, G):
     """res = D psi with a thread walking 1, 4 or 8 slices of the slowest
-    coordinate (t in 4D, z in 3D), and for the 4D FP32 and FP64 operators with
-    LDS-staged box tiles (march 16, 256 threads): every shape agrees with the oracle and
+    coordinate (t in 4D, z in 3D), and for the 4D FP32, FP64 and sloppy-FP64
+    operators with LDS-staged box tiles (march 16, 256 threads): every shape agrees with the oracle and
     with the one-site-per-thread result, and leaves the source checkerboard of
     res untouched."""
     print(f"[march] nd={nd} latt={latt} prec={prec} G={G}", flush=True)
@@ -284,7 +284,7 @@ def test_marching_lengths(oracle, nd, latt, prec, G):
     lib = _lib.core()
     b0, r0 = C.c_int(), C.c_int()
     _lib.check(lib.dslash_get_launch(nd, prec, G, C.byref(b0), C.byref(r0)))
-    marches = (1, 4, 8, 16) if (nd == 4 and prec in (F32, F64)) else (1, 4, 8)
+    marches = (1, 4, 8, 16) if (nd == 4 and prec in (F32, F64, F64S)) else (1, 4, 8)
     with _March(nd, prec, G) as m:
         for isign, cb in ((1, 0), (-1, 1)):
             want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)

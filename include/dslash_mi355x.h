@@ -135,8 +135,9 @@ int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int
  * the y / z hops inside the box (links non-temporal).  Any lattice extent is
  * allowed (the last chunk is shorter).  A launch that would have fewer than
  * 131072 threads at this length (small lattices, short t or z) uses the next
- * shorter one, down to 1; box tiles that cannot run fall back to 8.  The x + a
- * D psi forms always run one site per thread.  dslash_set_launch leaves it
+ * shorter one, down to 1; box tiles that cannot run (or would hold fewer than
+ * 64 sites) fall back to 8.  The x + a D psi forms run in box tiles under 16
+ * and one site per thread otherwise.  dslash_set_launch leaves it
  * unchanged; dslash_tune_apply tunes it with the block shape. */
 int dslash_set_launch_march(int nd, int prec, int gauge_reals, int march);
 int dslash_get_launch_march(int nd, int prec, int gauge_reals, int* march);

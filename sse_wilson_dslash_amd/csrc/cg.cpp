@@ -51,6 +51,7 @@ struct CgSolver {
   void *r = nullptr, *p = nullptr, *Ap = nullptr, *q = nullptr, *t = nullptr /* odd */, *xlo = nullptr;
   double* scal = nullptr;
   double* partial = nullptr;
+  size_t nparts = 0;         // (re, im) pairs the partial buffer holds
   double* h_scal = nullptr;  // pinned
   hipStream_t stream = nullptr;
   hipGraphExec_t g_iter = nullptr, g_pupd = nullptr;
@@ -100,9 +101,11 @@ static int apply_normal(const CgSolver* s, void* out, const void* in, void* t_od
 // One inner iteration: Ap = A p ; alpha = rr / <p,Ap> ; x += alpha p ; r -= alpha Ap ; RRNEW = |r|^2
 // <p, A p> = |Mhat p|^2 comes from the epilogue of the Dslash that forms Mhat p.
 static int sloppy_iteration(CgSolver* s) {
+  const int nb = apply_blocks(s->latt, 4, s->ps, s->Gs);
+  WD_REQUIRE(nb >= 1 && (size_t)nb <= s->nparts, "launch shape gives %d |Mhat p|^2 partial sums; the solver holds %zu",
+             nb, s->nparts);
   if (int e = apply_normal(s, s->Ap, s->p, s->t, s->q, s->ps, s->Gs, s->gs, s->gs_cb, s->partial)) return e;
-  if (int e = blas_finalize(s->partial, apply_blocks(s->latt, 4, s->ps, s->Gs), s->scal, FIN_ALPHA, s->stream))
-    return e;
+  if (int e = blas_finalize(s->partial, nb, s->scal, FIN_ALPHA, s->stream)) return e;
   return blas_cg_update({s->xlo, s->px}, {s->p, s->ps}, {s->r, s->ps}, {s->Ap, s->ps}, s->vh, s->partial, s->scal,
                         s->stream);
 }
@@ -173,7 +176,9 @@ void* dslash_cg_create(const int latt[4], const dslash_cg_params* prm) {
   const size_t b64 = spinor_cb_bytes(s->vh, F64), bs = spinor_cb_bytes(s->vh, s->ps), bx = spinor_cb_bytes(s->vh, s->px);
   s->Gh = prm->gauge_reals_precise == 12 ? 12 : 18;
   s->g64_cb = gauge_cb_bytes(s->vh, 4, F64, s->Gh);
+  // one partial per block (>= 64 sites) or box tile (>= 64 sites), or per reduction block
   const size_t nparts = (size_t)((s->vh + 63) / 64) > (size_t)RED_BLOCKS ? (size_t)((s->vh + 63) / 64) : (size_t)RED_BLOCKS;
+  s->nparts = nparts;
   s->gs_cb = gauge_cb_bytes(s->vh, 4, s->ps, s->Gs);
   bool ok = hipStreamCreate(&s->stream) == hipSuccess && !cg_alloc(&s->g64, 2 * s->g64_cb) &&
             !cg_alloc(&s->r_hi, b64) && !cg_alloc(&s->q_hi, b64) && !cg_alloc(&s->Ay_hi, b64) &&

@@ -23,6 +23,12 @@ namespace wd {
 enum Prec : int { F32 = 0, F64 = 1, F16 = 2, BF16 = 3, F64S = 4 };
 constexpr int NUM_PREC = 5;
 
+// Epilogues of the single-field operator: EPI_PLAIN res = D psi; EPI_XPAY
+// res = x + a D psi; EPI_XPAY_NORM also writes one (|res|^2, 0) partial sum per
+// block (one-site kernel) or per tile (box tiles), so the solvers get
+// <p, Mhat^dag Mhat p> = |Mhat p|^2 without another pass.
+enum { EPI_PLAIN = 0, EPI_XPAY = 1, EPI_XPAY_NORM = 2 };
+
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 template <int P> struct Fmt;

@@ -10,8 +10,9 @@ namespace wd {
 struct LaunchCfg {
   int block;  // threads per block
   int remap;  // XCD-contiguous block order
-  int march;  // res = D psi: slices of the slowest coordinate per thread (1, 4 or 8), or
-              // 16 = LDS-staged box tiles (4D FP32)
+  int march;  // slices of the slowest coordinate per thread (1, 4 or 8; the x + a D psi
+              // forms take 1), or 16 = LDS-staged box tiles (4D FP32, FP64, sloppy
+              // FP64; every form)
 };
 inline bool march_ok(int m) { return m == 1 || m == 4 || m == 8 || m == 16; }
 LaunchCfg launch_cfg(int nd, int prec, int G);
@@ -20,7 +21,9 @@ void set_launch_cfg(int nd, int prec, int G, LaunchCfg c);
 // res_cb = D psi_cb (xp_cb == nullptr) or res_cb = xp_cb + a * D psi_cb, all on
 // one checkerboard in the device layout of `prec`; xp_cb may alias res_cb.
 // With norm_partial (xpay form only) the kernel also writes one (|res|^2, 0)
-// pair per block; apply_blocks() gives the number of blocks.
+// pair per block (one site per thread) or per tile (box tiles);
+// apply_blocks() gives their number for the current launch shape (at most
+// ceil(Vh / 64)).
 int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd, int prec, int G,
                 int isign, int tcb, const void* xp_cb, double a, hipStream_t s, double* norm_partial = nullptr);
 int apply_blocks(const int latt[4], int nd, int prec, int G);

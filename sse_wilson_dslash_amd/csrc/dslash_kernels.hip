@@ -139,10 +139,8 @@ __device__ __forceinline__ void hop_dir(const ApplyArgs& a, const Coord& c, uint
   }
 }
 
-// Epilogues: EPI_PLAIN res = D psi; EPI_XPAY res = x + a D psi; EPI_XPAY_NORM also
-// writes the block's partial sum of |res|^2 to norm_partial[blockIdx.x] (the
-// solvers get <p, Mhat^dag Mhat p> = |Mhat p|^2 this way without another pass).
-enum { EPI_PLAIN = 0, EPI_XPAY = 1, EPI_XPAY_NORM = 2 };
+// Epilogues (EPI_*): the |res|^2 partial sum of EPI_XPAY_NORM goes to
+// norm_partial[blockIdx.x].
 
 // Minimum waves per SIMD the register allocator must leave room for: the
 // 16-bit formats have small loads that the scheduler would otherwise hoist
@@ -357,8 +355,9 @@ static int effective_march(const int latt[4], int nd, int prec, int block, int m
 
 int apply_blocks(const int latt[4], int nd, int prec, int G) {
   const uint32_t vh = (uint32_t)((uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2);
-  const int blk = launch_cfg(nd, prec, G).block;
-  return (int)((vh + blk - 1) / blk);
+  const LaunchCfg lc = launch_cfg(nd, prec, G);
+  if (effective_march(latt, nd, prec, lc.block, lc.march) == 16) return (int)box_tiles_single(latt, lc.block);
+  return (int)((vh + lc.block - 1) / lc.block);
 }
 
 int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd, int prec, int G,
@@ -392,11 +391,9 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
   a.slice = a.g.vh / (uint32_t)latt[nd - 1];
   a.dslice = make_fastdiv(a.slice);
   a.march = effective_march(latt, nd, prec, lc.block, lc.march);
-  if (a.march == 16 && !xp_cb) {  // LDS-staged box tiles
-    if (int e = box_apply_single(res_cb, psi_cb, gauge_cb, latt, prec, G, isign, tcb, lc.block, s)) return e;
-    return 0;
-  }
-  if (a.march == 16) a.march = 1;  // the x + a D psi forms run one site per thread
+  if (a.march == 16)  // LDS-staged box tiles, every epilogue form
+    return box_apply_single(res_cb, psi_cb, gauge_cb, latt, prec, G, isign, tcb, lc.block,
+                            BoxEpi{a.xp, xa, norm_partial}, s);
   if (nd == 4)
     launch_dslash_nd<4>(a, prec, G, isign, s);
   else

@@ -75,12 +75,23 @@ inline bool mrhs_box_compatible(const dslash_mrhs_problem* probs, int nprob, int
   return true;
 }
 int mrhs_launch(const MrhsList& list, int prec, int G, int R, int block, int t_block, uint32_t grid, hipStream_t s);
-// res = D psi for one FP32, FP64 or sloppy-FP64 4D field pair with box tiles and
-// non-temporal links (the single-RHS path's march 16); needs Lx/2 <= block and
-// block >= 128.
+
+// Epilogue operands of the single-field box launch: with xp set, res = xp + a D psi
+// (xp may alias res); with norm_partial also set, tile t writes its (|res|^2, 0)
+// pair to norm_partial[2 t], norm_partial[2 t + 1].
+struct BoxEpi {
+  const v4u* xp;
+  double a;
+  double* norm_partial;
+};
+// res = D psi (or an epilogue form) for one FP32, FP64 or sloppy-FP64 4D field
+// pair with box tiles and non-temporal links (the single-RHS path's march 16);
+// needs box_tiles_single(latt, block) > 0 and block >= 128.
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
-                     int isign, int tcb, int block, hipStream_t s);
-// Blocks (tiles) of that launch; 0 when Lx/2 > block.
+                     int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s);
+// Tiles (= blocks and |res|^2 partial sums) of that launch; 0 when Lx/2 > block
+// or when a box would hold fewer than 64 sites (then the tiles would outnumber
+// Vh / 64, the one-site kernel's most blocks, and the lanes would mostly idle).
 uint32_t box_tiles_single(const int latt[4], int block);
 
 }  // namespace wd

@@ -390,9 +390,12 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
 // One box tile: box (bty, btz) of slices t0 .. t0+7 (clipped to T), fields
 // r0 .. r0+R-1.  Every thread executes the same barriers.  NTL: links loaded
 // non-temporally (one field: every link is used once), else cached for the
-// other RHS blocks of the same box.
-template <int P, int G, int IS, int R, bool NTL>
-__device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0, int r0) {
+// other RHS blocks of the same box.  EPI (one field only): EPI_XPAY stores
+// x + a D psi, EPI_XPAY_NORM also adds each of the lane's |res|^2 to s2.
+template <int P, int G, int IS, int R, bool NTL, int EPI>
+__device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0, int r0,
+                              const BoxEpi& e, double& s2) {
+  static_assert(EPI == EPI_PLAIN || R == 1, "the epilogue forms take one field");
   typedef typename Fmt<P>::compute T;
   constexpr int TB = MRHS_T_BLOCK_LDS;
   constexpr int PV = PairVec<P, G>::value;
@@ -470,16 +473,49 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
       __builtin_amdgcn_sched_barrier(0);
       box_hop<P, G, IS, R, 2, NTL>(L, here, b, c, idx, tS, r0, acc);
       __builtin_amdgcn_sched_barrier(0);
+      if constexpr (EPI == EPI_PLAIN) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) store_spinor<P, true>(L.res + (size_t)(r0 + r) * L.fstride, nullptr, idx, vh, acc[r]);
+        for (int r = 0; r < R; ++r)
+          store_spinor<P, true>(L.res + (size_t)(r0 + r) * L.fstride, nullptr, idx, vh, acc[r]);
+      } else {  // read x before the store: x may alias res (same site, same lane)
+        const Spinor<T> x = box_spinor<P>(e.xp + idx, vh);
+        const T sa = (T)e.a;
+        T n2 = (T)0;
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            acc[0].s[sp][cc].re = x.s[sp][cc].re + sa * acc[0].s[sp][cc].re;
+            acc[0].s[sp][cc].im = x.s[sp][cc].im + sa * acc[0].s[sp][cc].im;
+            n2 += acc[0].s[sp][cc].re * acc[0].s[sp][cc].re + acc[0].s[sp][cc].im * acc[0].s[sp][cc].im;
+          }
+        if constexpr (EPI == EPI_XPAY_NORM) s2 += (double)n2;
+        store_spinor<P, false>(L.res, nullptr, idx, vh, acc[0]);
+      }
     }
     __syncthreads();  // buffer cur ^ 1 complete; buffer cur free for the next step's writes
   }
   __syncthreads();  // the next tile's preamble overwrites buffer 0
 }
 
-template <int P, int G, int R, int IS, bool NTL>
-__global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list) {
+// Sum of v over the block (<= 256 threads) to *out on thread 0.  Every thread
+// calls it; the caller's next barrier orders the LDS reuse of a following call.
+__device__ inline void box_block_sum(double v, double* out) {
+  __shared__ double part[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+    out[0] = t;
+    out[1] = 0.0;
+  }
+}
+
+template <int P, int G, int R, int IS, bool NTL, int EPI = EPI_PLAIN>
+__global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list, BoxEpi e) {
   extern __shared__ v4u box_slab[];
   const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
   const uint32_t total = list.total_tiles;
@@ -498,7 +534,10 @@ __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list) {
     const uint32_t q2 = q / L.nsb, box = q - q2 * L.nsb;
     const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
     const uint32_t btz = box / L.nty, bty = box - btz * L.nty;
-    mrhs_box_tile<P, G, IS, R, NTL>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS, (int)(grp * L.rg + rbi) * R);
+    double s2 = 0.0;
+    mrhs_box_tile<P, G, IS, R, NTL, EPI>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS,
+                                         (int)(grp * L.rg + rbi) * R, e, s2);
+    if constexpr (EPI == EPI_XPAY_NORM) box_block_sum(s2, e.norm_partial + 2 * (size_t)t);
   }
 }
 
@@ -553,15 +592,15 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
     const bool plus = list.lat[0].isign > 0;
     const size_t lds = (size_t)2 * R * spinor_vec(P) * block * sizeof(v4u);
     if constexpr (P == F64) {  // FP64: one field per thread
-      if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list);
-      else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list);
+      if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
+      else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
     } else {
       if (R == 1) {
-        if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list);
-        else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list);
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
+        else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
       } else {
-        if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 2, 1, false>), dim3(grid), dim3(block), lds, s, list);
-        else hipLaunchKernelGGL((k_mrhs_box<P, G, 2, -1, false>), dim3(grid), dim3(block), lds, s, list);
+        if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 2, 1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
+        else hipLaunchKernelGGL((k_mrhs_box<P, G, 2, -1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
       }
     }
   } else if constexpr (P == F32) {  // MRHS_T_BLOCK_LDS (validated by mrhs_shape_ok; one isign per list)
@@ -669,23 +708,37 @@ uint32_t box_tiles_single(const int latt[4], int block) {
   if (lxh > block) return 0;
   uint32_t ty, tz;
   box_shape(lxh, latt[1], latt[2], block, &ty, &tz);
+  if ((uint32_t)lxh * ty * tz < 64) return 0;
   const uint32_t nbox = ((latt[1] + ty - 1) / ty) * ((latt[2] + tz - 1) / tz);
   return nbox * (uint32_t)((latt[3] + MRHS_T_BLOCK_LDS - 1) / MRHS_T_BLOCK_LDS);
 }
 
-template <int P, int G>
-static void launch_box_single(const MrhsList& list, int isign, int block, hipStream_t s) {
+template <int P, int G, int IS>
+static void launch_box_single_s(const MrhsList& list, int block, const BoxEpi& e, hipStream_t s) {
   const size_t lds = (size_t)2 * spinor_vec(P) * block * sizeof(v4u);
   const dim3 grid(list.total_tiles), blk(block);
-  if (isign > 0)
-    hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, true>), grid, blk, lds, s, list);
+  if (!e.xp)
+    hipLaunchKernelGGL((k_mrhs_box<P, G, 1, IS, true, EPI_PLAIN>), grid, blk, lds, s, list, e);
+  else if (!e.norm_partial)
+    hipLaunchKernelGGL((k_mrhs_box<P, G, 1, IS, true, EPI_XPAY>), grid, blk, lds, s, list, e);
   else
-    hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, true>), grid, blk, lds, s, list);
+    hipLaunchKernelGGL((k_mrhs_box<P, G, 1, IS, true, EPI_XPAY_NORM>), grid, blk, lds, s, list, e);
+}
+
+template <int P, int G>
+static void launch_box_single(const MrhsList& list, int isign, int block, const BoxEpi& e, hipStream_t s) {
+  if (isign > 0)
+    launch_box_single_s<P, G, 1>(list, block, e, s);
+  else
+    launch_box_single_s<P, G, -1>(list, block, e, s);
 }
 
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
-                     int isign, int tcb, int block, hipStream_t s) {
+                     int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s) {
   WD_REQUIRE(prec == F32 || prec == F64 || prec == F64S, "box tiles are built for F32, F64 and F64S");
+  WD_REQUIRE(block >= 128 && box_tiles_single(latt, block) > 0,
+             "box tiles need >= 128 threads, Lx/2 <= block_threads and >= 64 sites per box");
+  WD_REQUIRE(!epi.norm_partial || epi.xp, "the |res|^2 epilogue needs the x + a*D psi form");
   dslash_mrhs_problem p;
   for (int mu = 0; mu < 4; ++mu) p.latt[mu] = latt[mu];
   p.d_gauge_cb = gauge_cb;
@@ -698,14 +751,14 @@ int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, con
   MrhsList list;
   if (int e = mrhs_build_list(&p, 1, prec, 1, block, MRHS_T_BOX, &list)) return e;
   if (prec == F32) {
-    if (G == 18) launch_box_single<F32, 18>(list, isign, block, s);
-    else launch_box_single<F32, 12>(list, isign, block, s);
+    if (G == 18) launch_box_single<F32, 18>(list, isign, block, epi, s);
+    else launch_box_single<F32, 12>(list, isign, block, epi, s);
   } else if (prec == F64) {
-    if (G == 18) launch_box_single<F64, 18>(list, isign, block, s);
-    else launch_box_single<F64, 12>(list, isign, block, s);
+    if (G == 18) launch_box_single<F64, 18>(list, isign, block, epi, s);
+    else launch_box_single<F64, 12>(list, isign, block, epi, s);
   } else {
-    if (G == 18) launch_box_single<F64S, 18>(list, isign, block, s);
-    else launch_box_single<F64S, 12>(list, isign, block, s);
+    if (G == 18) launch_box_single<F64S, 18>(list, isign, block, epi, s);
+    else launch_box_single<F64S, 12>(list, isign, block, epi, s);
   }
   WD_LAUNCHED();
   return 0;

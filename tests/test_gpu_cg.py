@@ -133,3 +133,74 @@ def test_initial_guess_is_used(oracle, problem):
     assert again["iterations"] == 0 and again["converged"]
     assert first["iterations"] > 10
     cg.close()
+
+
+class _OneSite:
+    """Runs the 4D operator of each (format, link reals) one site per thread
+    (march 1) inside the block, and restores the marching lengths after it."""
+
+    def __init__(self, *keys):
+        import ctypes as C
+        from sse_wilson_dslash_amd import _lib
+        self.lib, self.check, self.keys = _lib.core(), _lib.check, keys
+        self.saved = []
+        for prec, G in keys:
+            m = C.c_int()
+            self.check(self.lib.dslash_get_launch_march(4, prec, G, C.byref(m)))
+            self.saved.append(m.value)
+
+    def __enter__(self):
+        for prec, G in self.keys:
+            self.check(self.lib.dslash_set_launch_march(4, prec, G, 1))
+        return self
+
+    def __exit__(self, *exc):
+        for (prec, G), m in zip(self.keys, self.saved):
+            self.check(self.lib.dslash_set_launch_march(4, prec, G, m))
+
+
+@pytest.mark.parametrize("prec,G", [(F64, 18), (F32, 12)])
+def test_cg_with_box_tiles(oracle, prec, G):
+    """At 32^3 x 64 the sloppy operator and the FP64 operator of the reliable
+    updates run LDS-staged box tiles (march 16), whose x + a D psi epilogue also
+    forms the |Mhat p|^2 partial sums (one per tile) that give alpha.  The solve
+    meets the tolerance on the oracle's FP64 true residual and takes the same
+    number of iterations (+-1) as the solve with one site per thread."""
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    latt = (32, 32, 32, 64)
+    print(f"[cg box] latt={latt} prec={prec} G={G}", flush=True)
+    for p, g in ((prec, G), (F64, 18)):
+        m = C.c_int()
+        _lib.check(_lib.core().dslash_launch_march_for(_lib.latt_arg(latt), 4, p, g, C.byref(m)))
+        assert m.value == 16, (p, g, m.value)
+    u = gen_gauge_host(latt, 4, seed=2)
+    b = gen_spinor_host(latt, 4, seed=2, field=5)
+    b[volume(latt) // 2:] = 0.0
+    op = WilsonDslash(latt, 4, F64)
+    d_b = op.pack_cb(b, 0)
+    iters = {}
+    for label in ("box", "one-site"):
+        d_x = op.new_spinor(full=False)
+        if label == "box":
+            cg = WilsonCG(latt, kappa=KAPPA, tol=1e-10, prec_sloppy=prec, gauge_reals_sloppy=G, delta=0.1)
+            cg.load_gauge(u)
+            r = cg.solve(d_x, d_b)
+        else:
+            with _OneSite((prec, G), (F64, 18)):
+                cg = WilsonCG(latt, kappa=KAPPA, tol=1e-10, prec_sloppy=prec, gauge_reals_sloppy=G, delta=0.1)
+                cg.load_gauge(u)
+                r = cg.solve(d_x, d_b)
+        cg.close()
+        x = op.unpack_cb(d_x, 0)
+        tr = true_residual(oracle, u, x, b, latt)
+        print(f"  {label}: iters={r['iterations']} reliable={r['reliable_updates']} "
+              f"solver_res={r['true_residual']:.3e} oracle_res={tr:.3e}", flush=True)
+        assert r["converged"] and r["true_residual"] <= 1e-10
+        assert tr <= 1.05e-10
+        assert abs(tr - r["true_residual"]) < 1e-12
+        iters[label] = r["iterations"]
+        del d_x
+    assert abs(iters["box"] - iters["one-site"]) <= 1, iters
+    del op, d_b
+    torch.cuda.empty_cache()

@@ -348,6 +348,68 @@ def test_xpay_epilogue(oracle, prec):
             assert np.array_equal(got_inplace, got)
 
 
+# t = 66 leaves a 2-slice last tile; 24 x 48 x 48 boxes (12 x 4 x 4 sites) leave
+# 64 lanes of each 256-thread block idle; t = 2 is shorter than a tile.
+XPAY_BOX_CASES = [((32, 32, 32, 66), F64, 18), ((24, 48, 48, 48), F32, 18), ((64, 64, 64, 2), F64S, 12),
+                  ((32, 32, 32, 66), F32, 12)]
+
+
+@pytest.mark.parametrize("latt,prec,G", XPAY_BOX_CASES)
+def test_xpay_epilogue_box_tiles(oracle, latt, prec, G):
+    """dslash_apply_xpay on lattices where the 4D operator runs LDS-staged box
+    tiles (march 16): res = x + a D psi agrees with the oracle and with the
+    one-site-per-thread kernel (march 1), and the in-place form (x aliasing res)
+    gives the same bits as the out-of-place one."""
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    from sse_wilson_dslash_amd._lib import ptr
+    print(f"[xpay box] latt={latt} prec={prec} G={G}", flush=True)
+    dt = np.float32 if prec == F32 else np.float64
+    u = gen_gauge_host(latt, 4, seed=4, dtype=dt)
+    psi = gen_spinor_host(latt, 4, seed=4, field=1, dtype=dt)
+    x = gen_spinor_host(latt, 4, seed=4, field=2, dtype=dt)
+    a = -0.0144
+    op = WilsonDslash(latt, 4, prec, G)
+    op.load_gauge(u)
+    vh = op.vh
+    lib = _lib.core()
+    b0, r0 = C.c_int(), C.c_int()
+    _lib.check(lib.dslash_get_launch(4, prec, G, C.byref(b0), C.byref(r0)))
+    stream = torch.cuda.current_stream().cuda_stream
+    try:
+        with _March(4, prec, G) as m:
+            for isign, tcb in ((1, 0), (-1, 1)):
+                d_psi = op.pack_cb(psi, 1 - tcb)
+                d_x = op.pack_cb(x, tcb)
+                g = ptr(op.gauge) + tcb * op.gb
+                t = slice(tcb * vh, (tcb + 1) * vh)
+                want = (x + a * oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, 1 - tcb)).reshape(-1, 24)[t]
+                outs = {}
+                for march in (16, 1):
+                    m.set(march)
+                    _lib.check(lib.dslash_set_launch(4, prec, G, 256, 1))
+                    assert _march_for(latt, 4, prec, G) == march
+                    d_res = op.new_spinor(False)
+                    _lib.check(lib.dslash_apply_xpay(ptr(d_res), ptr(d_psi), g, ptr(d_x), a, op._L, prec, G, isign,
+                                                     tcb, stream))
+                    d_in = d_x.clone()
+                    _lib.check(lib.dslash_apply_xpay(ptr(d_in), ptr(d_psi), g, ptr(d_in), a, op._L, prec, G, isign,
+                                                     tcb, stream))
+                    got = op.unpack_cb(d_res, tcb, dtype=dt).reshape(-1, 24)[t]
+                    got_in = op.unpack_cb(d_in, tcb, dtype=dt).reshape(-1, 24)[t]
+                    e2, einf = R.rel_err(got, want), R.max_rel_err(got, want)
+                    print(f"  isign={isign:+d} tcb={tcb} march={march}: rel L2 {e2:.2e} rel max {einf:.2e}",
+                          flush=True)
+                    assert e2 < MARCH_TOL[prec] and einf < MARCH_TOL[prec], (march, isign, tcb, e2, einf)
+                    assert np.array_equal(got_in, got), (march, isign, tcb)
+                    outs[march] = got
+                assert np.max(np.abs(outs[16] - outs[1])) <= MARCH_SAME[prec] * np.max(np.abs(outs[1]))
+    finally:
+        _lib.check(lib.dslash_set_launch(4, prec, G, b0.value, r0.value))
+    del op
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("p16,tol", [(F16, 6e-4), (BF16, 4e-3)])
 def test_precision_conversion_round_trip(p16, tol):
     """F64 -> 16-bit (per-site norm) -> F64 keeps every site to the format's rounding

@@ -50,6 +50,9 @@ class Stager {
   }
 
   void finish() {
+    // the last uploads may still be reading the buffers
+    (void)reuse(0);
+    (void)reuse(1);
     stop_workers();
     for (hipEvent_t& e : ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;

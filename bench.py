@@ -46,6 +46,31 @@ def measured_traffic(name: str):
     return float(m.group(1)) if m else None
 
 
+def fp64_result(latt, nd: int, seed: int, src, isign: int, src_cb: int):
+    """D psi of the FP64 operator (equal to the oracle to 1e-13 in the GPU tests) on
+    the seeded gauge for a host spinor src, as a full-lattice float64 array with the
+    target checkerboard filled."""
+    import numpy as np
+    import torch
+    from sse_wilson_dslash_amd import F64
+    from sse_wilson_dslash_amd.device import WilsonDslash
+
+    ref = WilsonDslash(latt, nd, F64, 18)
+    ref.gen_gauge(seed=seed)
+    psi = ref.pack_spinor(np.ascontiguousarray(src, dtype=np.float64))
+    res = ref.new_spinor(zero=True)
+    ref.apply(res, psi, isign, src_cb)
+    want = ref.unpack_spinor(res, dtype=np.float64, cbs=(1 - src_cb,))
+    del ref, psi, res
+    torch.cuda.empty_cache()
+    return want
+
+
+def rel_err(got, want) -> float:
+    import numpy as np
+    return float(np.linalg.norm(got - want) / np.linalg.norm(want))
+
+
 def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=True):
     import torch
     from sse_wilson_dslash_amd import F32, F64
@@ -76,9 +101,17 @@ def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=Tr
         "wall_s": wall,
         "launch": {"block_threads": shape["block_threads"], "xcd_remap": shape["xcd_remap"], "march": shape["march"]},
     }
+    err = ""
+    if prec == F32:  # error of the timed launch shape against the FP64 operator, same seeded fields
+        import numpy as np
+        src = op.unpack_spinor(psi, dtype=np.float64, cbs=(0,))
+        op.apply(res, psi, 1, 0)
+        out["rel_err_vs_fp64"] = rel_err(op.unpack_spinor(res, dtype=np.float64, cbs=(1,)),
+                                         fp64_result(LATT, nd, 1, src, 1, 0))
+        err = f", rel. error vs FP64 operator {out['rel_err_vs_fp64']:.2e}"
     if verbose:
         log(f"[gpu] {prec_name} nd={nd} G={G}: {ms * 1e3:.1f} us/call, {out['gflops']:.0f} GFLOP/s, "
-            f"{out['gbs']:.0f} GB/s ({out['gbs'] / PEAK_HBM_GBS:.1%} of HBM peak), launch {out['launch']}")
+            f"{out['gbs']:.0f} GB/s ({out['gbs'] / PEAK_HBM_GBS:.1%} of HBM peak), launch {out['launch']}{err}")
     del op, psi, res
     torch.cuda.empty_cache()
     return out
@@ -359,6 +392,8 @@ def mrhs_leg(reps: int = 50):
                                             tune_batch)
     from sse_wilson_dslash_amd.device import WilsonDslash
 
+    import numpy as np
+
     def item(latt, nrhs, seed):
         op = WilsonDslash(latt, 4, F32, 18)
         op.gen_gauge(seed=seed)
@@ -383,7 +418,15 @@ def mrhs_leg(reps: int = 50):
     ms = timed(items, 0)
     out["rhs32_24c48"] = {"ms": round(ms, 4), "gflops": round(flops(items) / (ms * 1e-3) / 1e9, 1),
                           "hbm_gbs": round(compulsory_bytes(items) / (ms * 1e-3) / 1e9, 1), "tuned": t}
-    log(f"[mrhs] 32 RHS 24^3x48: {ms * 1e3:.1f} us/launch, {out['rhs32_24c48']['gflops']:.0f} GFLOP/s, tuned {t}")
+    errs = []
+    for k in (0, 31):  # fields of the timed launch against the FP64 operator on the same sources
+        op, psi, res = items[0].op, items[0].psi, items[0].res
+        src = op.unpack_cb(psi.field(k), 1, dtype=np.float64)
+        errs.append(rel_err(op.unpack_cb(res.field(k), 0, dtype=np.float64),
+                            fp64_result((24, 24, 24, 48), 4, 1, src, 1, 1)))
+    out["rhs32_24c48"]["rel_err_vs_fp64"] = max(errs)
+    log(f"[mrhs] 32 RHS 24^3x48: {ms * 1e3:.1f} us/launch, {out['rhs32_24c48']['gflops']:.0f} GFLOP/s, tuned {t}, "
+        f"rel. error vs FP64 operator (fields 0, 31) {max(errs):.2e}")
     del items
     items = [item((8, 8, 8, 8), 4, 2), item((16, 16, 16, 32), 4, 3), item((24, 24, 24, 48), 4, 4)]
     t = tune_batch(items, persistent=True)
@@ -471,7 +514,8 @@ def main() -> int:
         gpu_leg(args.steps, args.warmup, "f32", 4, 18)
         return 0
     head = gpu_leg(args.steps, args.warmup, "f32", 4, 18)
-    extra = {"copy": copy_leg()}
+    extra = {"copy": copy_leg(),
+             "f32_4d": {"rel_err_vs_fp64": head.get("rel_err_vs_fp64"), "launch": head["launch"]}}
     if not args.no_extra:
         n = max(200, args.steps // 4)
         for key, (p, nd, G) in {"f64_4d": ("f64", 4, 18), "f32_4d_g12": ("f32", 4, 12),
@@ -480,6 +524,8 @@ def main() -> int:
             extra[key] = {"gflops": round(r["gflops"], 1), "us_per_call": round(r["ms"] * 1e3, 2),
                           "hbm_gbs": round(r["gbs"], 1), "hbm_frac": round(r["gbs"] / PEAK_HBM_GBS, 4),
                           "launch": r["launch"]}
+            if "rel_err_vs_fp64" in r:
+                extra[key]["rel_err_vs_fp64"] = r["rel_err_vs_fp64"]
         for p in ("f16", "bf16"):
             extra[f"{p}_4d_g12"] = half_leg(n, p)
         extra["f64_sloppy_4d"] = half_leg(n, "f64_sloppy", 18)

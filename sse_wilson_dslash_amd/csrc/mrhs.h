@@ -89,9 +89,9 @@ struct BoxEpi {
 // needs box_tiles_single(latt, block) > 0 and block >= 128.
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
                      int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s);
-// Tiles (= blocks and |res|^2 partial sums) of that launch; 0 when Lx/2 > block
-// or when a box would hold fewer than 64 sites (then the tiles would outnumber
-// Vh / 64, the one-site kernel's most blocks, and the lanes would mostly idle).
+// Tiles (= blocks and |res|^2 partial sums) of that launch; 0 when Lx/2 > block,
+// when a box would hold fewer than 64 sites (the lanes would mostly idle) or when
+// the tiles would outnumber ceil(Vh / 64), the one-site kernel's most blocks.
 uint32_t box_tiles_single(const int latt[4], int block);
 
 }  // namespace wd

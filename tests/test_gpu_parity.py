@@ -256,7 +256,8 @@ def test_marching_falls_back_on_small_lattices(oracle, nd, latt, prec):
 # z = 34 leave a 2-slice last chunk at march 4 and 8; t = 2 and z = 2 are shorter
 # than the march; t = 65 is an odd time extent of the 3D operator.
 MARCH_CASES = [(4, (32, 32, 32, 66), F64, 18), (4, (32, 32, 32, 66), F32, 12), (4, (32, 32, 32, 66), BF16, 12),
-               (4, (24, 48, 48, 48), F32, 18),  # box tiles of 12 x 4 x 4 sites: 64 idle lanes per block
+               (4, (24, 48, 48, 48), F32, 18),  # box tiles of 12 x 3 x 6 sites: 40 idle lanes per block
+               (4, (48, 48, 48, 24), F64, 18),  # box tiles of 24 x 3 x 3 sites: 40 idle lanes per block
                (4, (64, 64, 64, 2), F64S, 18), (4, (64, 64, 64, 2), F16, 12),
                (3, (32, 32, 34, 65), F64, 12), (3, (32, 32, 34, 65), F32, 18), (3, (32, 32, 34, 65), F16, 18),
                (3, (64, 64, 2, 64), BF16, 18), (3, (64, 64, 2, 64), F64S, 12)]
@@ -348,8 +349,8 @@ def test_xpay_epilogue(oracle, prec):
             assert np.array_equal(got_inplace, got)
 
 
-# t = 66 leaves a 2-slice last tile; 24 x 48 x 48 boxes (12 x 4 x 4 sites) leave
-# 64 lanes of each 256-thread block idle; t = 2 is shorter than a tile.
+# t = 66 leaves a 2-slice last tile; 24 x 48 x 48 boxes (12 x 3 x 6 sites) leave
+# 40 lanes of each 256-thread block idle; t = 2 is shorter than a tile.
 XPAY_BOX_CASES = [((32, 32, 32, 66), F64, 18), ((24, 48, 48, 48), F32, 18), ((64, 64, 64, 2), F64S, 12),
                   ((32, 32, 32, 66), F32, 12)]
 

@@ -71,7 +71,7 @@ def rel_err(got, want) -> float:
     return float(np.linalg.norm(got - want) / np.linalg.norm(want))
 
 
-def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=True):
+def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=True, check_err=True):
     import torch
     from sse_wilson_dslash_amd import F32, F64
     from sse_wilson_dslash_amd.device import WilsonDslash
@@ -102,7 +102,7 @@ def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=Tr
         "launch": {"block_threads": shape["block_threads"], "xcd_remap": shape["xcd_remap"], "march": shape["march"]},
     }
     err = ""
-    if prec == F32:  # error of the timed launch shape against the FP64 operator, same seeded fields
+    if prec == F32 and check_err:  # error of the timed launch shape against the FP64 operator, same fields
         import numpy as np
         src = op.unpack_spinor(psi, dtype=np.float64, cbs=(0,))
         op.apply(res, psi, 1, 0)
@@ -511,7 +511,7 @@ def main() -> int:
         return 0
     if args.pmc_workload:
         copy_leg(2.0, 3)
-        gpu_leg(args.steps, args.warmup, "f32", 4, 18)
+        gpu_leg(args.steps, args.warmup, "f32", 4, 18, check_err=False)
         return 0
     head = gpu_leg(args.steps, args.warmup, "f32", 4, 18)
     extra = {"copy": copy_leg(),

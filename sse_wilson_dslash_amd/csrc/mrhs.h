@@ -14,7 +14,8 @@ constexpr int MRHS_T_BLOCK = 4;      // t-slices per thread in t-marching tiles
 constexpr int MRHS_T_BLOCK_LDS = 8;  // t-slices per block in LDS-staged t-marching tiles
 constexpr int MRHS_LDS_THREADS = 128;  // threads (= spatial sites) per LDS-staged block
 constexpr int MRHS_LDS_GROUP = 8;    // RHS blocks of one spatial block run together
-constexpr int MRHS_T_BOX = 16;       // t_block value of LDS-staged box tiles (8 slices per tile)
+constexpr int MRHS_T_BOX = 16;       // t_block value of LDS-staged box tiles
+constexpr int MRHS_BOX_SLICES = 8;   // t-slices per box tile
 
 // Tile shapes: t_block 1 (one site per thread, any rhs_block), MRHS_T_BLOCK
 // (t-marching, rhs_block <= 2), MRHS_T_BLOCK_LDS (t-marching with an LDS slab,

@@ -387,7 +387,7 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
   }
 }
 
-// One box tile: box (bty, btz) of slices t0 .. t0+7 (clipped to T), fields
+// One box tile: box (bty, btz) of slices t0 .. t0 + MRHS_BOX_SLICES - 1 (clipped to T), fields
 // r0 .. r0+R-1.  Every thread executes the same barriers.  NTL: links loaded
 // non-temporally (one field: every link is used once), else cached for the
 // other RHS blocks of the same box.  EPI (one field only): EPI_XPAY stores
@@ -397,7 +397,7 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
                               const BoxEpi& e, double& s2) {
   static_assert(EPI == EPI_PLAIN || R == 1, "the epilogue forms take one field");
   typedef typename Fmt<P>::compute T;
-  constexpr int TB = MRHS_T_BLOCK_LDS;
+  constexpr int TB = MRHS_BOX_SLICES;
   constexpr int PV = PairVec<P, G>::value;
   constexpr int NV = 24 * Fmt<P>::bytes / 16;
   const uint32_t vh = L.g.vh, S = L.S, nt = blockDim.x;
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list, BoxEpi e) {
     const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
     const uint32_t btz = box / L.nty, bty = box - btz * L.nty;
     double s2 = 0.0;
-    mrhs_box_tile<P, G, IS, R, NTL, EPI>(L, box_slab, bty, btz, (int)tc * MRHS_T_BLOCK_LDS,
+    mrhs_box_tile<P, G, IS, R, NTL, EPI>(L, box_slab, bty, btz, (int)tc * MRHS_BOX_SLICES,
                                          (int)(grp * L.rg + rbi) * R, e, s2);
     if constexpr (EPI == EPI_XPAY_NORM) box_block_sum(s2, e.norm_partial + 2 * (size_t)t);
   }
@@ -689,7 +689,7 @@ int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R
       box_shape(L.g.lxh, p.latt[1], p.latt[2], block, &L.ty, &L.tz);
       L.nty = (p.latt[1] + L.ty - 1) / L.ty;
       L.nsb = L.nty * ((p.latt[2] + L.tz - 1) / L.tz);
-      L.ntc = (p.latt[3] + MRHS_T_BLOCK_LDS - 1) / MRHS_T_BLOCK_LDS;
+      L.ntc = (p.latt[3] + MRHS_BOX_SLICES - 1) / MRHS_BOX_SLICES;
       if (L.nrb > MRHS_LDS_GROUP && L.nrb % MRHS_LDS_GROUP == 0) L.rg = MRHS_LDS_GROUP;
     } else {
       L.nsb = (L.S + block - 1) / block;
@@ -712,7 +712,7 @@ uint32_t box_tiles_single(const int latt[4], int block) {
   box_shape(lxh, latt[1], latt[2], block, &ty, &tz);
   if ((uint32_t)lxh * ty * tz < 64) return 0;
   const uint64_t nbox = (uint64_t)((latt[1] + ty - 1) / ty) * (uint64_t)((latt[2] + tz - 1) / tz);
-  const uint64_t tiles = nbox * (uint64_t)((latt[3] + MRHS_T_BLOCK_LDS - 1) / MRHS_T_BLOCK_LDS);
+  const uint64_t tiles = nbox * (uint64_t)((latt[3] + MRHS_BOX_SLICES - 1) / MRHS_BOX_SLICES);
   const uint64_t vh = (uint64_t)lxh * latt[1] * latt[2] * latt[3];
   return tiles <= (vh + 63) / 64 ? (uint32_t)tiles : 0;
 }

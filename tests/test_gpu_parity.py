@@ -5,6 +5,9 @@ the largest entry): FP64 1e-13, FP32 1e-5.  The 16-bit formats are a storage
 compression of the operator used inside the mixed-precision solver; their
 error against the FP64 oracle is bounded and reported, not a parity claim.
 """
+import glob
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -64,6 +67,29 @@ def test_small_lattices(oracle, nd, latt, prec, G):
             check(got, want, latt, cb, prec, f"nd={nd} {latt} isign={isign} cb={cb}")
             vh = volume(latt) // 2
             assert np.all(got.reshape(-1, 24)[cb * vh:(cb + 1) * vh] == 5.5), "source cb of res modified"
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: p.rsplit("/", 1)[-1][:-4])
+@pytest.mark.parametrize("prec,G", [(F64, 18), (F32, 18), (F64, 12), (F32, 12)])
+def test_golden_fixtures(path, prec, G):
+    """The HIP path against the committed oracle fixtures (tests/golden), without
+    calling the oracle: both isign, both source checkerboards, 18- and 12-real
+    links; the source checkerboard of res keeps its sentinel."""
+    f = np.load(path)
+    nd, latt = int(f["nd"]), tuple(int(v) for v in f["latt"])
+    tag, dt = ("f64", np.float64) if prec == F64 else ("f32", np.float32)
+    print(f"[golden] {path.rsplit('/', 1)[-1]} prec={prec} G={G}", flush=True)
+    u, psi = f["u"].astype(dt), f["psi"].astype(dt)
+    vh = volume(latt) // 2
+    for isign in (1, -1):
+        for cb in (0, 1):
+            got = run_gpu(u, psi, latt, nd, prec, G, isign, cb, np.full_like(psi, 2.5))
+            check(got, f[f"res_{tag}_{'p' if isign > 0 else 'm'}{cb}"], latt, cb, prec,
+                  f"{path} isign={isign} cb={cb}")
+            assert np.all(got.reshape(-1, 24)[cb * vh:(cb + 1) * vh] == 2.5), "source cb of res modified"
 
 
 @pytest.mark.parametrize("prec", [F64, F32])

@@ -8,6 +8,9 @@ gauge covariance and brute-force layout round trips.  Each check is chosen so
 that a plausible slip (wrong sign of one projector row, a transposed link, a
 swapped neighbour, an off-by-one in the checkerboard order) fails it.
 """
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -216,3 +219,31 @@ def test_layout_round_trip(oracle, nd, latt):
         oracle.lib().oracle_coords(L, nd, i, out)
         assert list(out) == list(c[i])
         assert oracle.lib().oracle_index(L, nd, out) == i
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "*.npz")))
+
+
+def test_golden_fixtures_present():
+    assert len(GOLDEN) == 3, GOLDEN
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: p.rsplit("/", 1)[-1][:-4])
+def test_golden_fixtures_reproduce(oracle, path):
+    """The host generators and the oracle still give the committed fixtures
+    (tests/golden/make_fixtures.py): inputs bit for bit, D psi to 1e-14 (FP64)
+    and 1e-6 (FP32) relative, source checkerboard zero."""
+    from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
+    f = np.load(path)
+    nd, latt, seed = int(f["nd"]), tuple(int(v) for v in f["latt"]), int(f["seed"])
+    assert np.array_equal(gen_gauge_host(latt, nd, seed=seed), f["u"])
+    assert np.array_equal(gen_spinor_host(latt, nd, seed=seed, field=1), f["psi"])
+    vh = volume(latt) // 2
+    for tag, dt, tol in (("f64", np.float64, 1e-14), ("f32", np.float32, 1e-6)):
+        u, psi = f["u"].astype(dt), f["psi"].astype(dt)
+        for isign in (1, -1):
+            for cb in (0, 1):
+                want = f[f"res_{tag}_{'p' if isign > 0 else 'm'}{cb}"]
+                got = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
+                assert R.rel_err(got, want) <= tol, (tag, isign, cb)
+                assert not want.reshape(-1, 24)[cb * vh:(cb + 1) * vh].any()

@@ -147,8 +147,8 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
         for (int mi = 0; mi < nm; ++mi) {
           // box tiles ignore the remap flag (their tile order is XCD-contiguous by
           // construction); tuning them with remap 1 only keeps the flag right for
-          // the one-site kernels that share this entry (x + a D psi forms, the
-          // domain-decomposition interior, lattices below the box thread floor)
+          // the one-site kernels that share this entry (the domain-decomposition
+          // interior, lattices below the box thread floor)
           if (marches[mi] == 16 && (blocks[bi] < 128 || remap == 0)) continue;
           set_launch_cfg(nd, prec, G, LaunchCfg{blocks[bi], remap, marches[mi]});
           float t = 0;

@@ -159,7 +159,7 @@ class _OneSite:
             self.check(self.lib.dslash_set_launch_march(4, prec, G, m))
 
 
-@pytest.mark.parametrize("prec,G", [(F64, 18), (F32, 12)])
+@pytest.mark.parametrize("prec,G", [(F64, 18), (F32, 12), (F32, 18)])
 def test_cg_with_box_tiles(oracle, prec, G):
     """At 32^3 x 64 the sloppy operator and the FP64 operator of the reliable
     updates run LDS-staged box tiles (march 16), whose x + a D psi epilogue also

@@ -62,8 +62,21 @@ int sse_su3dslash_wilson_resident(SSEREAL* psi, SSEREAL* res, int isign, int cb)
 
 /* Fully device-resident: d_psi/d_res are full-lattice fields in the device
  * layout of dslash_mi355x.h in this library's precision; stream is a
- * hipStream_t.  Asynchronous. */
+ * hipStream_t.  Asynchronous.  A later gauge upload, gauge reuse rebuild or
+ * free_sse_su3dslash waits for the kernels these calls queued on `stream`
+ * before it rewrites or frees the resident links. */
 int sse_su3dslash_wilson_device(void* d_res, const void* d_psi, int isign, int cb, void* stream);
+
+/* Gauge reuse for sse_su3dslash_wilson (opt-in; the reference uploads nothing
+ * and so has no such switch).  mode 0 (default): the gauge is uploaded on every
+ * call, as the reference reads it on every call.  mode 1: the packed links of
+ * both checkerboards stay in HBM and are uploaded again only when u is a
+ * different pointer or a 64-bit checksum of the whole array (computed on the
+ * host threads each call) differs from the last upload's.  mode 2: reuse
+ * whenever u is the same pointer; the caller promises not to change the array
+ * in place, or calls sse_su3dslash_gauge_release() after changing it.  Either
+ * mode replaces a gauge set with sse_su3dslash_gauge_upload. */
+int sse_su3dslash_set_gauge_reuse(int mode);
 
 /* Sloppy FP64 (FP64 library only; DSLASH_ERR_UNSUPPORTED in the FP32 one):
  * on = 1 keeps FP64 spinors at the interface but packs links in FP32 and runs
@@ -79,8 +92,13 @@ int sse_su3dslash_set_sloppy(int on);
  * [cb][Vh][mu] order, without transposing.  volume = V/2. */
 typedef SSEREAL sse_u_mat_array[3][3][2];
 void pack_gauge_field(int volume, sse_u_mat_array* u, sse_u_mat_array* u_tmp);
-/* Inverse of pack_gauge_field. */
+/* The reference's unpack_gauge_field (lib/packer_nopad.c:47-69), byte for byte:
+ * it copies u_tmp to u position by position in [mu][cb][volume] order, so u
+ * becomes a copy of u_tmp (it does NOT undo pack_gauge_field). */
 void unpack_gauge_field(int volume, sse_u_mat_array* u_tmp, sse_u_mat_array* u);
+/* Extension (no counterpart in the reference): the true inverse of
+ * pack_gauge_field, [cb][volume][mu] back to [mu][cb][volume]. */
+void unpack_gauge_field_inverse(int volume, sse_u_mat_array* u_tmp, sse_u_mat_array* u);
 
 #ifdef __cplusplus
 }

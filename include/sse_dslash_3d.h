@@ -39,6 +39,8 @@ int sse_su3dslash_wilson_3d_resident(SSEREAL* psi, SSEREAL* res, int isign, int 
 int sse_su3dslash_wilson_3d_device(void* d_res, const void* d_psi, int isign, int cb, void* stream);
 /* Sloppy FP64 (FP64 library only), as sse_su3dslash_set_sloppy. */
 int sse_su3dslash_3d_set_sloppy(int on);
+/* Gauge reuse for sse_su3dslash_wilson_3d, as sse_su3dslash_set_gauge_reuse. */
+int sse_su3dslash_3d_set_gauge_reuse(int mode);
 
 #ifdef __cplusplus
 }

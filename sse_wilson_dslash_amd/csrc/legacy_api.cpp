@@ -61,6 +61,17 @@ struct Legacy {
   int resident_G = 0;        // 0: nothing resident; 18/12: both bundles hold the uploaded gauge
   int prec = LEG_DEV_PREC;   // device format of bundles and kernel (LEG_PREC or F64_SLOPPY)
   hipStream_t stream = nullptr;
+  // *_device calls run on the caller's stream and read d_bundle after the call
+  // returns; dev_done marks the last of them, and everything that rewrites or
+  // frees d_bundle waits for it first.
+  hipEvent_t dev_done = nullptr;
+  bool dev_pending = false;
+  // Gauge reuse for the host-pointer call (0 off, 1 pointer + checksum, 2 pointer):
+  // the resident bundles then hold the gauge last uploaded from reuse_ptr.
+  int reuse_mode = 0;
+  bool reuse_valid = false;
+  const SSEREAL* reuse_ptr = nullptr;
+  uint64_t reuse_sum = 0;
   std::vector<SSEREAL> h_stage;
   wd::Stager stager;
 
@@ -163,6 +174,7 @@ struct Legacy {
     gb18 = dslash_gauge_cb_bytes(latt, nd, LEG_PREC, 18);
     hipError_t e;
     if ((e = hipStreamCreate(&stream)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&dev_done, hipEventDisableTiming)) != hipSuccess ||
         (e = hipMalloc(&d_u_aos, V * 4 * 18 * RS)) != hipSuccess ||
         (e = hipMalloc(&d_psi_aos, V * 24 * RS)) != hipSuccess ||
         (e = hipMalloc(&d_res_aos, V * 24 * RS)) != hipSuccess ||
@@ -183,6 +195,8 @@ struct Legacy {
     }
     stager.start();
     resident_G = 0;
+    dev_pending = false;
+    reuse_valid = false;
     refcount = 1;
     last_error = 0;
   }
@@ -191,6 +205,8 @@ struct Legacy {
     if (refcount == 0) return;  // lib/sse_su3dslash_32bit_scalar.c:127-129
     if (--refcount > 0) return;
     if (stream) (void)hipStreamSynchronize(stream);
+    if (dev_pending) (void)hipEventSynchronize(dev_done);
+    dev_pending = false;
     stager.finish();
     void* ptrs[] = {d_u_aos, d_psi_aos, d_res_aos, d_psi_cb, d_res_cb, d_bundle, d_bundle_call, d_site_table};
     for (void* p : ptrs)
@@ -199,9 +215,23 @@ struct Legacy {
     d_site_table = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
+    if (dev_done) (void)hipEventDestroy(dev_done);
+    dev_done = nullptr;
     site_table.clear();
     h_stage.clear();
     resident_G = 0;
+    reuse_valid = false;
+  }
+
+  // Order the internal stream after every *_device launch issued so far: the
+  // next kernel that rewrites the resident bundles must not overtake a kernel
+  // on the caller's stream that still reads them.
+  int after_device_calls() {
+    if (!dev_pending) return 0;
+    hipError_t e = hipStreamWaitEvent(stream, dev_done, 0);
+    if (e != hipSuccess) return hip_fail(e, "wait for device-resident calls");
+    dev_pending = false;
+    return 0;
   }
 
   int check_ready() {
@@ -217,6 +247,7 @@ struct Legacy {
   int upload_gauge(const SSEREAL* u, int G, void* const slot[2]) {
     hipError_t e = stager.h2d(d_u_aos, u, V * 4 * 18 * RS, stream);
     if (e != hipSuccess) return hip_fail(e, "gauge upload");
+    if (int r = after_device_calls()) return r;
     for (int t = 0; t < 2; ++t)
       if (slot[t] && dslash_pack_gauge(slot[t], t, d_u_aos, LEG_PREC, latt, nd, prec, G, d_site_table, stream))
         return fail(dslash_last_error(), "gauge pack");
@@ -259,6 +290,7 @@ struct Legacy {
     if (isign != 1 && isign != -1) return 0;  // lib/sse_su3dslash_32bit_scalar.c:162-179 does nothing
     if (int r = check_ready()) return r;
     if (cb != 0 && cb != 1) return fail(DSLASH_ERR_INVALID, "cb must be 0 or 1");
+    if (reuse_mode) return wilson_reuse(u, psi, res, isign, cb);
     // The gauge may change between calls, so it is uploaded every call and
     // only the target checkerboard's bundle is built.
     const int tcb = 1 - cb;
@@ -272,6 +304,8 @@ struct Legacy {
     if (G != 18 && G != 12) return fail(DSLASH_ERR_INVALID, "gauge_reals must be 18 or 12");
     size_t gb = dslash_gauge_cb_bytes(latt, nd, prec, G);
     void* slots[2] = {d_bundle, (char*)d_bundle + gb};
+    resident_G = 0;
+    reuse_valid = false;
     if (int r = upload_gauge(u, G, slots)) return r;
     hipError_t e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "synchronize");
@@ -282,6 +316,36 @@ struct Legacy {
 
   int gauge_release() {
     resident_G = 0;
+    reuse_valid = false;
+    return 0;
+  }
+
+  // Host-pointer call with gauge reuse: the bundles of both target
+  // checkerboards stay resident and are rebuilt only when u is a different
+  // pointer than last time, or (mode 1) its checksum changed.
+  int wilson_reuse(SSEREAL* u, SSEREAL* psi, SSEREAL* res, int isign, int cb) {
+    const size_t ub = V * 4 * 18 * RS;
+    const uint64_t sum = reuse_mode == 1 ? stager.checksum(u, ub) : 0;
+    if (!(reuse_valid && resident_G == 18 && u == reuse_ptr && (reuse_mode == 2 || sum == reuse_sum))) {
+      reuse_valid = false;
+      resident_G = 0;
+      const size_t gb = dslash_gauge_cb_bytes(latt, nd, prec, 18);
+      void* slots[2] = {d_bundle, (char*)d_bundle + gb};
+      if (int r = upload_gauge(u, 18, slots)) return r;
+      resident_G = 18;
+      reuse_valid = true;
+      reuse_ptr = u;
+      reuse_sum = sum;
+    }
+    const size_t gb = dslash_gauge_cb_bytes(latt, nd, prec, 18);
+    return apply_host(psi, res, isign, cb, (char*)d_bundle + (size_t)(1 - cb) * gb, 18);
+  }
+
+  int set_gauge_reuse(int mode) {
+    if (mode < 0 || mode > 2) return fail(DSLASH_ERR_INVALID, "gauge reuse mode must be 0, 1 or 2");
+    if (mode != reuse_mode) reuse_valid = false;
+    reuse_mode = mode;
+    last_error = 0;
     return 0;
   }
 
@@ -290,7 +354,7 @@ struct Legacy {
   int set_sloppy(int on) {
     if (on && LEG_PREC != DSLASH_F64) return fail(DSLASH_ERR_UNSUPPORTED, "sloppy mode exists for the FP64 library only");
     int p = on ? DSLASH_F64_SLOPPY : LEG_PREC;
-    if (p != prec) resident_G = 0;
+    if (p != prec) resident_G = 0, reuse_valid = false;
     prec = p;
     last_error = 0;
     return 0;
@@ -310,6 +374,9 @@ struct Legacy {
     if (!resident_G) return fail(DSLASH_ERR_NOT_INIT, "no resident gauge: call *_gauge_upload first");
     if (dslash_apply_full(d_res, d_psi, d_bundle, latt, nd, prec, resident_G, isign, cb, s))
       return fail(dslash_last_error(), "dslash");
+    hipError_t e = hipEventRecord(dev_done, (hipStream_t)s);
+    if (e != hipSuccess) return hip_fail(e, "record device-call event");
+    dev_pending = true;
     last_error = 0;
     return 0;
   }
@@ -342,6 +409,7 @@ int sse_su3dslash_wilson_device(void* d_res, const void* d_psi, int isign, int c
   return g4.wilson_device(d_res, d_psi, isign, cb, stream);
 }
 int sse_su3dslash_set_sloppy(int on) { return g4.set_sloppy(on); }
+int sse_su3dslash_set_gauge_reuse(int mode) { return g4.set_gauge_reuse(mode); }
 
 void init_sse_su3dslash_3d(const int* latt_size, void (*getSiteCoords)(int coord[], int node, int linearsite),
                            int (*getLinearSiteIndex)(const int coord[]), int (*nodeNumber)(const int coord[])) {
@@ -362,6 +430,7 @@ int sse_su3dslash_wilson_3d_device(void* d_res, const void* d_psi, int isign, in
   return g3.wilson_device(d_res, d_psi, isign, cb, stream);
 }
 int sse_su3dslash_3d_set_sloppy(int on) { return g3.set_sloppy(on); }
+int sse_su3dslash_3d_set_gauge_reuse(int mode) { return g3.set_gauge_reuse(mode); }
 
 // lib/packer_nopad.c:29-45: u is [mu][cb][volume], u_tmp is [cb][volume][mu].
 void pack_gauge_field(int volume, sse_u_mat_array* u, sse_u_mat_array* u_tmp) {
@@ -372,7 +441,19 @@ void pack_gauge_field(int volume, sse_u_mat_array* u, sse_u_mat_array* u_tmp) {
                sizeof(sse_u_mat_array));
 }
 
+// lib/packer_nopad.c:47-69: despite its name the reference's unpacker reads
+// and writes the same [mu][cb][ix] positions (two sites at a time through a
+// stack buffer), so the caller's u ends up a copy of u_tmp.  The same bytes are
+// produced here.  (For an odd volume the reference's pairwise loop also touches
+// the entry one past each [mu][cb] block, which for the last block is past the
+// arrays; with every global dimension even, volume = V/2 is a multiple of 8.)
 void unpack_gauge_field(int volume, sse_u_mat_array* u_tmp, sse_u_mat_array* u) {
+  if (volume <= 0 || u == u_tmp) return;
+  memmove(u, u_tmp, (size_t)8 * (size_t)volume * sizeof(sse_u_mat_array));
+}
+
+// The true inverse of pack_gauge_field: [cb][Vh][mu] back to [mu][cb][Vh].
+void unpack_gauge_field_inverse(int volume, sse_u_mat_array* u_tmp, sse_u_mat_array* u) {
   for (int cb = 0; cb < 2; ++cb)
     for (int ix = 0; ix < volume; ++ix)
       for (int mu = 0; mu < 4; ++mu)

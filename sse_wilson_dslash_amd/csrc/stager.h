@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stdint.h>
 #include <string.h>
 
 #include <algorithm>
@@ -77,6 +78,18 @@ class Stager {
     return hipSuccess;
   }
 
+  // 64-bit checksum of n host bytes, computed by the calling thread and the
+  // workers in parallel (a change detector for a host array, not a
+  // cryptographic hash).  The value depends on the worker count, which is
+  // fixed between start() and finish().
+  uint64_t checksum(const void* h, size_t n) {
+    if (workers.empty() || n < 2 * CHUNK) return digest((const char*)h, n);
+    run(nullptr, (const char*)h, n);
+    uint64_t r = (uint64_t)job_parts;
+    for (int k = 0; k < job_parts; ++k) r = (r ^ part_sum[k]) * 0x9E3779B97F4A7C15ull, r ^= r >> 31;
+    return r;
+  }
+
   // Downloads after the work already on s; returns with h complete.
   hipError_t d2h(void* h, const void* d, size_t n, hipStream_t s) {
     if (!usable() || n < 2 * CHUNK) {
@@ -127,7 +140,11 @@ class Stager {
   }
 
   // memcpy split over the workers and the calling thread
-  void copy(char* dst, const char* src, size_t n) {
+  void copy(char* dst, const char* src, size_t n) { run(dst, src, n); }
+
+  // One job (copy when dst != nullptr, else checksum) split over the workers
+  // and the calling thread.
+  void run(char* dst, const char* src, size_t n) {
     const int parts = (int)workers.size() + 1;
     {
       std::lock_guard<std::mutex> lk(m);
@@ -146,7 +163,38 @@ class Stager {
 
   void part(int k) {
     const size_t a = job_n * k / job_parts, b = job_n * (k + 1) / job_parts;
-    memcpy(job_dst + a, job_src + a, b - a);
+    if (job_dst)
+      memcpy(job_dst + a, job_src + a, b - a);
+    else
+      part_sum[k] = digest(job_src + a, b - a);
+  }
+
+  // 64-bit digest of a byte range: four independent lanes, each folding one
+  // 8-byte word per step through xor, a multiply by an odd constant and an
+  // xor-shift (each a bijection of the lane state), then a tail and length fold.
+  static uint64_t digest(const char* p, size_t n) {
+    uint64_t h[4] = {0x243F6A8885A308D3ull, 0x13198A2E03707344ull, 0xA4093822299F31D0ull, 0x082EFA98EC4E6C89ull};
+    auto mix = [](uint64_t x, uint64_t w) {
+      x = (x ^ w) * 0x9E3779B97F4A7C15ull;
+      return x ^ (x >> 29);
+    };
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+      uint64_t w[4];
+      memcpy(w, p + i, 32);
+      h[0] = mix(h[0], w[0]);
+      h[1] = mix(h[1], w[1]);
+      h[2] = mix(h[2], w[2]);
+      h[3] = mix(h[3], w[3]);
+    }
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      memcpy(&w, p + i, 8);
+      h[0] = mix(h[0], w);
+    }
+    uint64_t t = 0;
+    for (int j = 0; i < n; ++i, ++j) t |= (uint64_t)(unsigned char)p[i] << (8 * j);
+    return mix(mix(mix(mix(mix(h[0], h[1]), h[2]), h[3]), t), (uint64_t)n);
   }
 
   void work(int k, unsigned seen) {
@@ -179,6 +227,7 @@ class Stager {
   const char* job_src = nullptr;
   size_t job_n = 0;
   int job_parts = 1;
+  uint64_t part_sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // one per part (at most 7 workers + caller)
 };
 
 }  // namespace wd

@@ -85,7 +85,8 @@ class SSEDslash:
                      "sse_su3dslash_gauge_release", "sse_su3dslash_3d_gauge_release"):
             getattr(L, name).restype = _i
             getattr(L, name).argtypes = []
-        for name in ("sse_su3dslash_set_sloppy", "sse_su3dslash_3d_set_sloppy"):
+        for name in ("sse_su3dslash_set_sloppy", "sse_su3dslash_3d_set_sloppy", "sse_su3dslash_set_gauge_reuse",
+                     "sse_su3dslash_3d_set_gauge_reuse"):
             getattr(L, name).restype = _i
             getattr(L, name).argtypes = [_i]
         for name in ("sse_su3dslash_gauge_upload", "sse_su3dslash_3d_gauge_upload"):
@@ -97,7 +98,7 @@ class SSEDslash:
         for name in ("sse_su3dslash_wilson_device", "sse_su3dslash_wilson_3d_device"):
             getattr(L, name).restype = _i
             getattr(L, name).argtypes = [_vp, _vp, _i, _i, _vp]
-        for name in ("pack_gauge_field", "unpack_gauge_field"):
+        for name in ("pack_gauge_field", "unpack_gauge_field", "unpack_gauge_field_inverse"):
             getattr(L, name).restype = None
             getattr(L, name).argtypes = [_i, _vp, _vp]
         self._keep = {}
@@ -166,6 +167,14 @@ class SSEDslash:
     def set_sloppy_3d(self, on: bool = True) -> int:
         return self.lib.sse_su3dslash_3d_set_sloppy(int(bool(on)))
 
+    def set_gauge_reuse(self, mode: int) -> int:
+        """0: upload the gauge every call; 1: reuse while pointer and checksum
+        match; 2: reuse while the pointer matches."""
+        return self.lib.sse_su3dslash_set_gauge_reuse(int(mode))
+
+    def set_gauge_reuse_3d(self, mode: int) -> int:
+        return self.lib.sse_su3dslash_3d_set_gauge_reuse(int(mode))
+
     # ---- 3D ------------------------------------------------------------
     def init_sse_su3dslash_3d(self, latt_size, getSiteCoords=None, getLinearSiteIndex=None, nodeNumber=None):
         self._init(self.lib.init_sse_su3dslash_3d, 3, latt_size, getSiteCoords, getLinearSiteIndex, nodeNumber, 3)
@@ -196,5 +205,11 @@ class SSEDslash:
         self.lib.pack_gauge_field(int(volume), self._arr(u, n, "u"), self._arr(u_tmp, n, "u_tmp"))
 
     def unpack_gauge_field(self, volume: int, u_tmp: np.ndarray, u: np.ndarray) -> None:
+        """The reference's unpacker (lib/packer_nopad.c:47-69): u becomes a copy of u_tmp."""
         n = 2 * volume * 4 * 18
         self.lib.unpack_gauge_field(int(volume), self._arr(u_tmp, n, "u_tmp"), self._arr(u, n, "u"))
+
+    def unpack_gauge_field_inverse(self, volume: int, u_tmp: np.ndarray, u: np.ndarray) -> None:
+        """Extension: the true inverse of pack_gauge_field."""
+        n = 2 * volume * 4 * 18
+        self.lib.unpack_gauge_field_inverse(int(volume), self._arr(u_tmp, n, "u_tmp"), self._arr(u, n, "u"))

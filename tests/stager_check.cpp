@@ -31,6 +31,30 @@ int main() {
         return 1;
       }
     }
+    // checksum: reproducible, and any single changed byte changes it (first,
+    // part boundaries, last full word, the tail), for the parallel path (n >=
+    // 64 MiB) and the single-thread one
+    for (size_t n : {N - 3, (size_t)1 << 20}) {
+      const uint64_t c0 = st.checksum(h.data(), n);
+      if (st.checksum(h.data(), n) != c0) {
+        printf("checksum not reproducible: size %zu\n", n);
+        return 1;
+      }
+      const size_t at[] = {0, n / 8, n / 2 + 1, n - 9, n - 1};
+      for (size_t i : at) {
+        h[i] ^= 0x10;
+        const uint64_t c1 = st.checksum(h.data(), n);
+        h[i] ^= 0x10;
+        if (c1 == c0) {
+          printf("checksum blind to byte %zu of %zu\n", i, n);
+          return 1;
+        }
+      }
+      if (st.checksum(h.data(), n - 1) == c0) {
+        printf("checksum blind to length\n");
+        return 1;
+      }
+    }
     st.finish();
   }
 #ifndef __SANITIZE_ADDRESS__

@@ -204,3 +204,69 @@ def test_cg_with_box_tiles(oracle, prec, G):
     assert abs(iters["box"] - iters["one-site"]) <= 1, iters
     del op, d_b
     torch.cuda.empty_cache()
+
+
+@pytest.fixture(scope="module")
+def problem_48c96():
+    """The mixed-precision CG workload at its own size: 48^3 x 96, seeded SU(3)
+    gauge (generated in HBM and copied to the host for the oracle) and a
+    Gaussian source on the even checkerboard."""
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    latt = (48, 48, 48, 96)
+    V = volume(latt)
+    d_u = torch.empty(V * 72, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.core().dslash_gen_gauge_device(_lib.ptr(d_u), F64, _lib.latt_arg(latt), 4, 1,
+                                                     torch.cuda.current_stream().cuda_stream))
+    u = d_u.cpu().numpy().reshape(V, 4, 3, 3, 2)
+    b = gen_spinor_host(latt, 4, seed=1, field=7)
+    b[V // 2:] = 0.0
+    yield latt, d_u, u, b
+    del d_u
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.timeout(900)
+def test_cg_48c96_mixed_vs_fp64(oracle, problem_48c96):
+    """Mixed-precision CG on Mhat^dag Mhat at 48^3 x 96, kappa = 0.12: F16 and
+    BF16 spinors with per-site norms, 12-real 16-bit links, FP32 arithmetic,
+    FP64 reliable updates (delta = 3e-3), graph-replayed iterations.  The FP64
+    true residual of each returned solution is recomputed on the host with the
+    oracle (<= 1.05e-10, agreeing with the solver's own value to 1e-12), and
+    each mixed solve is at least 2x faster than plain FP64 CG (18-real FP64
+    links, no reliable updates but the final one) on the same source.
+    delta is per format: a reliable update is due once |r| has dropped by
+    delta, which must stay well above the format's rounding error (2^-11 for
+    F16, 2^-8 for BF16); at delta = 3e-3 the BF16 inner residual drifts and the
+    solve takes ~80 iterations instead of ~22."""
+    latt, d_u, u, b = problem_48c96
+    op = WilsonDslash(latt, 4, F64)
+    d_b = op.pack_cb(b, 0)
+    runs = (("fp64", F64, 18, 0.0), ("mixed_f16", F16, 12, 3e-3), ("mixed_bf16", BF16, 12, 1e-2))
+    ms = {}
+    for name, prec, G, delta in runs:
+        print(f"[cg 48^3x96] {name}", flush=True)
+        cg = WilsonCG(latt, kappa=KAPPA, tol=1e-10, prec_sloppy=prec, gauge_reals_sloppy=G, delta=delta,
+                      use_graph=True, gauge_reals_precise=18)
+        cg.load_gauge(d_u)
+        d_x = op.new_spinor(full=False)
+        cg.solve(d_x, d_b)  # warm-up: graph capture, first touch
+        r = cg.solve(d_x, d_b)
+        cg.close()
+        ms[name] = r["seconds"] * 1e3
+        print(f"  {r['seconds'] * 1e3:.1f} ms iters={r['iterations']} reliable={r['reliable_updates']} "
+              f"solver_res={r['true_residual']:.3e}", flush=True)
+        assert r["converged"] and r["true_residual"] <= 1e-10
+        if prec != F64:
+            assert r["reliable_updates"] >= 1
+            x = op.unpack_cb(d_x, 0)
+            tr = true_residual(oracle, u, x, b, latt)
+            print(f"  oracle_res={tr:.3e}", flush=True)
+            assert tr <= 1.05e-10
+            assert abs(tr - r["true_residual"]) < 1e-12
+            del x
+        del d_x
+        torch.cuda.empty_cache()
+    for name in ("mixed_f16", "mixed_bf16"):
+        print(f"  speed-up {name}: {ms['fp64'] / ms[name]:.2f}x", flush=True)
+        assert ms["fp64"] >= 2.0 * ms[name], ms

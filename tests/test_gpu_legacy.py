@@ -250,3 +250,99 @@ def test_legacy_semantics(oracle, libs):
         lg.free_sse_su3dslash()
     # after the last free the API reports use-before-init instead of crashing
     assert lg.wilson_resident(psi, np.zeros_like(psi), 1, 0) != 0
+
+
+def test_device_call_then_upload_on_side_stream(libs):
+    """sse_su3dslash_wilson_device on a non-default stream returns before its
+    kernels run; a gauge upload issued right after must not rewrite the
+    resident links under them.  Many queued applications of gauge A, then an
+    upload of gauge B: the result still equals D(A) psi bit for bit, and the
+    next application uses B."""
+    from sse_wilson_dslash_amd import F32
+    from sse_wilson_dslash_amd.device import WilsonDslash
+    lg = libs[32]
+    latt = (32, 32, 32, 64)
+    print(f"[legacy side stream] latt={latt}", flush=True)
+    ua = gen_gauge_host(latt, 4, seed=1, dtype=np.float32)
+    ub = gen_gauge_host(latt, 4, seed=2, dtype=np.float32)
+    psi = gen_spinor_host(latt, 4, seed=1, dtype=np.float32)
+    op = WilsonDslash(latt, 4, F32, 18)
+    d_psi = op.pack_spinor(psi)
+    lg.init_sse_su3dslash(latt)
+    try:
+        assert lg.gauge_upload(ua, 18) == 0
+        want_a = op.new_spinor(zero=True)
+        assert lg.wilson_device(want_a, d_psi, 1, 0, torch.cuda.current_stream().cuda_stream) == 0
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        d_res = op.new_spinor(zero=True)
+        torch.cuda.synchronize()
+        for _ in range(200):
+            assert lg.wilson_device(d_res, d_psi, 1, 0, side.cuda_stream) == 0
+        assert lg.gauge_upload(ub, 18) == 0  # must wait for the 200 launches above
+        side.synchronize()
+        assert torch.equal(d_res, want_a)
+        assert lg.wilson_device(d_res, d_psi, 1, 0, side.cuda_stream) == 0
+        side.synchronize()
+        op.load_gauge(ub)
+        want_b = op.new_spinor(zero=True)
+        op.apply(want_b, d_psi, 1, 0)
+        torch.cuda.synchronize()
+        assert torch.equal(d_res, want_b) and not torch.equal(want_a, want_b)
+    finally:
+        lg.free_sse_su3dslash()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_gauge_reuse_modes(oracle, libs, prec):
+    """sse_su3dslash_set_gauge_reuse: mode 1 reuses the uploaded links while the
+    pointer and checksum are unchanged and notices an in-place change; mode 2
+    reuses while the pointer is unchanged (an in-place change is seen only after
+    gauge_release, as documented); a different array is always uploaded; mode 0
+    is the reference's upload on every call.  16^3 x 32 FP64 (75 MB of links)
+    takes the multi-threaded checksum, 8^4 the single-thread one."""
+    lg = libs[prec]
+    dt = lg.dtype
+    tol = 1e-5 if prec == 32 else 1e-13
+    for latt in ((8, 8, 8, 8), (16, 16, 16, 32)):
+        print(f"[legacy reuse] prec={prec} latt={latt}", flush=True)
+        u = gen_gauge_host(latt, 4, seed=1, dtype=dt)
+        u2 = gen_gauge_host(latt, 4, seed=2, dtype=dt)
+        psi = gen_spinor_host(latt, 4, seed=1, dtype=dt)
+        d1 = oracle.dslash(u, psi, np.zeros_like(psi), latt, 1, 0)
+        d2 = oracle.dslash(u2, psi, np.zeros_like(psi), latt, 1, 0)
+        lg.init_sse_su3dslash(latt)
+
+        def call(arr, isign=1, cb=0):
+            res = np.zeros_like(psi)
+            lg.sse_su3dslash_wilson(arr, psi, res, isign, cb)
+            assert lg.last_error() == 0
+            return res
+
+        try:
+            assert lg.set_gauge_reuse(3) != 0 and lg.set_gauge_reuse(-1) != 0
+            assert lg.set_gauge_reuse(1) == 0
+            work = u.copy()
+            assert R.rel_err(call(work), d1) < tol
+            assert R.rel_err(call(work), d1) < tol  # reused
+            r = call(work, -1, 1)                   # both checkerboards are resident
+            assert R.rel_err(r, oracle.dslash(u, psi, np.zeros_like(psi), latt, -1, 1)) < tol
+            work[...] = u2                          # in place: the checksum changes
+            assert R.rel_err(call(work), d2) < tol
+            work.reshape(-1)[5] += 0.25             # a single real changed in place
+            d3 = oracle.dslash(work, psi, np.zeros_like(psi), latt, 1, 0)
+            assert R.rel_err(call(work), d3) < tol
+            assert R.rel_err(call(u), d1) < tol     # another pointer
+            assert lg.set_gauge_reuse(2) == 0
+            work[...] = u
+            assert R.rel_err(call(work), d1) < tol
+            work[...] = u2                          # pointer unchanged: still the old links
+            assert R.rel_err(call(work), d1) < tol
+            assert lg.gauge_release() == 0          # the caller says it changed
+            assert R.rel_err(call(work), d2) < tol
+            assert lg.set_gauge_reuse(0) == 0
+            work[...] = u
+            assert R.rel_err(call(work), d1) < tol
+        finally:
+            lg.set_gauge_reuse(0)
+            lg.free_sse_su3dslash()

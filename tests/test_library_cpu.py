@@ -379,21 +379,79 @@ def test_staging_pool(tmp_path, sanitizer):
     assert out.returncode == 0 and out.stdout.startswith("stager ok"), out.stdout + out.stderr[-2000:]
 
 
+def _ref_unpack_gauge_field(volume, u_tmp, u):
+    """lib/packer_nopad.c:47-69 step by step: for each cb, sites in pairs
+    (ix, ix+1), the four directions' matrices at [mu][cb][ix] are read into a
+    stack buffer v[8] and written back to the same [mu][cb][ix] positions of u."""
+    src = u_tmp.reshape(-1, 18)
+    dst = u.reshape(-1, 18)
+    for cb in range(2):
+        for ix in range(0, volume, 2):
+            v = [None] * 8
+            for mu in range(4):
+                v[2 * mu] = src[ix + volume * (cb + 2 * mu)].copy()
+                v[2 * mu + 1] = src[ix + 1 + volume * (cb + 2 * mu)].copy()
+            for mu in range(4):
+                dst[ix + volume * (cb + 2 * mu)] = v[2 * mu]
+                dst[ix + 1 + volume * (cb + 2 * mu)] = v[2 * mu + 1]
+
+
 @pytest.mark.parametrize("prec", [32, 64])
 def test_c_gauge_packer(prec):
     """pack_gauge_field reorders [mu][cb][Vh] -> [cb][Vh][mu] without transposing
-    (lib/packer_nopad.c:29-45); unpack_gauge_field is its inverse."""
+    (lib/packer_nopad.c:29-45).  unpack_gauge_field is byte-identical to the
+    reference's (lib/packer_nopad.c:47-69, restated step by step above, which
+    leaves u a copy of u_tmp); unpack_gauge_field_inverse undoes the pack.
+    Integer-valued fills, so any misplaced matrix or element shows."""
     from sse_wilson_dslash_amd.legacy import SSEDslash
     lg = SSEDslash(prec)
-    vh = 12
     dt = np.float32 if prec == 32 else np.float64
-    u = np.arange(4 * 2 * vh * 18, dtype=dt).reshape(4, 2, vh, 18)
-    out = np.zeros_like(u).reshape(2, vh, 4, 18)
-    lg.pack_gauge_field(vh, u, out)
-    assert np.array_equal(out, u.transpose(1, 2, 0, 3))
-    back = np.zeros_like(u)
-    lg.unpack_gauge_field(vh, out, back)
-    assert np.array_equal(back, u)
+    for vh in (12, 8, 128):
+        u = np.arange(4 * 2 * vh * 18, dtype=dt).reshape(4, 2, vh, 18)
+        out = np.zeros_like(u).reshape(2, vh, 4, 18)
+        lg.pack_gauge_field(vh, u, out)
+        assert np.array_equal(out, u.transpose(1, 2, 0, 3))
+        back = np.zeros_like(u)
+        lg.unpack_gauge_field_inverse(vh, out, back)
+        assert np.array_equal(back, u)
+        # the reference's unpacker: same bytes as the step-by-step restatement
+        got = np.full_like(u, -1.0)
+        want = np.full_like(u, -1.0)
+        lg.unpack_gauge_field(vh, out, got)
+        _ref_unpack_gauge_field(vh, out, want)
+        assert got.tobytes() == want.tobytes()
+        assert np.array_equal(got.reshape(-1), out.reshape(-1))
+        # in place (u_tmp == u), as a caller may pass the same array
+        same = out.copy()
+        lg.unpack_gauge_field(vh, same, same)
+        assert same.tobytes() == out.tobytes()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_config_header_without_define(tmp_path, prec):
+    """A caller compiled with the reference's include line and no -DSSE_PRECISION
+    (include/sse_config.h:13 takes it from sse_config_internal.h) builds, links
+    against the library of that precision and runs; SSEREAL has the library's
+    width."""
+    src = tmp_path / "cfg.c"
+    src.write_text('#include "sse_dslash.h"\n#include "sse_dslash_3d.h"\n#include <stdio.h>\n'
+                   "int main(void) {\n"
+                   "  SSEREAL u[2 * 8 * 4 * 18], v[2 * 8 * 4 * 18];\n"
+                   "  for (int i = 0; i < 2 * 8 * 4 * 18; ++i) u[i] = (SSEREAL)i;\n"
+                   "  pack_gauge_field(8, (sse_u_mat_array*)u, (sse_u_mat_array*)v);\n"
+                   "  unpack_gauge_field_inverse(8, (sse_u_mat_array*)v, (sse_u_mat_array*)u);\n"
+                   "  for (int i = 0; i < 2 * 8 * 4 * 18; ++i) if (u[i] != (SSEREAL)i) return 2;\n"
+                   '  printf("SSEREAL %d %d\\n", (int)sizeof(SSEREAL), SSE_PRECISION);\n'
+                   "  return sse_su3dslash_last_error();\n}\n")
+    exe = tmp_path / "cfg"
+    libdir = os.path.join(ROOT, "sse_wilson_dslash_amd", "_lib")
+    cmd = ["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(ROOT, "include", f"f{prec}"), "-o", str(exe), str(src),
+           "-L", libdir, f"-lsse_dslash_f{prec}", f"-Wl,-rpath,{libdir}"]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.split() == ["SSEREAL", str(prec // 8), str(prec)]
 
 
 @pytest.mark.parametrize("nd,latt,msg", [(4, "[4,4,3,4]", b"4 dimensions"), (3, "[4,3,4,5]", b"3 dimensions")])

@@ -9,9 +9,11 @@ Gram-Schmidt-reunitarised complex Gaussians (seed 1) and N(0,1) spinors,
 generated on the device.
 
 Also reported (same JSON line): the roofline of the dominant kernel measured
-with HIP events on its launch stream, the oracle's CPU throughput on the host
-cores (a bounded sample of the same lattice), and secondary figures (FP64, 3D,
-12-real links).
+with HIP events on its launch stream, the CPU baseline (the host-CPU build of
+the same drop-in API, libsse_dslash_cpu_f32.so, on all usable host cores and
+on one core; a bounded sample of the same lattice), and secondary figures
+(FP64, 3D, 12-real links, 16-bit and sloppy formats, batched launches, the
+domain-decomposition split, the backward kernel, the CG, the drop-in API).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S]
 """
@@ -117,36 +119,96 @@ def gpu_leg(steps: int, warmup: int, prec_name: str, nd: int, G: int, verbose=Tr
     return out
 
 
-def cpu_leg(seconds: float, threads: int):
-    """The oracle (plain C + OpenMP) on the host cores, same lattice, FP32."""
-    import numpy as np
-    from oracle import oracle
-    from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
+def host_cpu_info() -> dict:
+    """Host CPUs this process may use: the affinity set, its physical cores
+    (distinct (package, core) pairs in /proc/cpuinfo) and the cgroup CPU quota."""
+    aff = sorted(os.sched_getaffinity(0))
+    info = {"affinity_cpus": len(aff)}
+    try:
+        cores, model, cur = set(), None, {}
+        for line in open("/proc/cpuinfo"):
+            if not line.strip():
+                if cur.get("processor") in aff:
+                    cores.add((cur.get("physical id", 0), cur.get("core id", cur.get("processor"))))
+                cur = {}
+                continue
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("processor", "physical id", "core id"):
+                cur[k] = int(v)
+            elif k == "model name" and model is None:
+                model = v
+        if cur.get("processor") in aff:
+            cores.add((cur.get("physical id", 0), cur.get("core id", cur.get("processor"))))
+        info["physical_cores"] = len(cores)
+        info["model"] = model
+    except OSError:
+        pass
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            info["cgroup_cpu_quota"] = round(int(q) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    return info
 
-    used = oracle.set_threads(threads)
-    u = gen_gauge_host(LATT, 4, seed=1, dtype=np.float32)
-    psi = gen_spinor_host(LATT, 4, seed=1, dtype=np.float32)
+
+def cpu_child(prec: int, seconds: float, nd: int = 4) -> None:
+    """Runs in a child process (OMP_* set by the parent): times the host-CPU
+    library (libsse_dslash_cpu_f{32,64}.so, the drop-in API computed with
+    OpenMP) at 32^3 x 64 and prints one JSON line."""
+    import numpy as np
+    from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
+    from sse_wilson_dslash_amd.legacy import SSEDslash
+
+    lg = SSEDslash(prec, backend="cpu")
+    u = gen_gauge_host(LATT, nd, seed=1, dtype=lg.dtype)
+    psi = gen_spinor_host(LATT, nd, seed=1, dtype=lg.dtype)
     res = np.zeros_like(psi)
-    oracle.dslash(u, psi, res, LATT, 1, 0)  # warm-up (first touch)
+    init, apply = ((lg.init_sse_su3dslash, lg.sse_su3dslash_wilson) if nd == 4 else
+                   (lg.init_sse_su3dslash_3d, lg.sse_su3dslash_wilson_3d))
+    init(LATT)
+    for k in range(3):
+        apply(u, psi, res, 1 if k % 4 < 2 else -1, k % 2)
     times = []
     t_start = time.perf_counter()
     k = 0
-    while True:
+    while k < 20 or (time.perf_counter() - t_start < seconds and k < 2000):
         t0 = time.perf_counter()
-        oracle.dslash(u, psi, res, LATT, 1 if k % 4 < 2 else -1, k % 2)
+        apply(u, psi, res, 1 if k % 4 < 2 else -1, k % 2)
         times.append(time.perf_counter() - t0)
         k += 1
-        if (time.perf_counter() - t_start > seconds and k >= 3) or k >= 2000:
-            break
     times.sort()
     med = times[len(times) // 2]
     vh = int(np.prod(LATT)) // 2
-    g = 1320 * vh / med / 1e9
-    wall = time.perf_counter() - t_start
-    log(f"[cpu] oracle f32 on {used} threads: {med * 1e3:.1f} ms/call, {g:.2f} GFLOP/s ({k} calls, {wall:.1f} s)")
-    return {"value": g, "unit": "GFLOP/s", "cores": used, "kind": "port",
-            "sample": f"{k} calls ({wall:.1f} s) of the FP32 4D oracle on 32x32x32x64 (one checkerboard each, "
-                      f"alternating isign and cb), median {med * 1e3:.1f} ms/call, OpenMP over target sites"}
+    fl = (1320 if nd == 4 else 984) * vh
+    print(json.dumps({"ms": med * 1e3, "gflops": fl / med / 1e9, "calls": k,
+                      "wall_s": time.perf_counter() - t_start}), flush=True)
+
+
+def cpu_leg(seconds: float, threads: int):
+    """The host-CPU build of the drop-in API (csrc/cpu_dslash.cpp: OpenMP, AVX2,
+    projector-specialised hops, blocked traversal) at 32^3 x 64, FP32 and FP64,
+    on all usable host cores and on one core, each in a child process with
+    OMP_NUM_THREADS, OMP_PROC_BIND=spread and OMP_PLACES=cores."""
+    import subprocess
+    host = host_cpu_info()
+    out = {"host": host, "threads_all": threads}
+    for prec in (32, 64):
+        for n in (threads, 1):
+            env = dict(os.environ, OMP_NUM_THREADS=str(n), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", str(prec),
+                                "--cpu-seconds", str(seconds if n > 1 else seconds / 2)],
+                               env=env, capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(f"CPU leg failed: {r.stderr[-2000:]}")
+            d = json.loads(r.stdout.strip().splitlines()[-1])
+            key = f"f{prec}_{'all' if n > 1 else '1'}_thread{'s' if n > 1 else ''}"
+            out[key] = {"ms_per_call": round(d["ms"], 2), "gflops": round(d["gflops"], 2), "calls": d["calls"],
+                        "threads": n}
+            log(f"[cpu] libsse_dslash_cpu_f{prec} on {n} threads: {d['ms']:.2f} ms/call, {d['gflops']:.1f} GFLOP/s "
+                f"({d['calls']} calls, {d['wall_s']:.1f} s)")
+    return out
 
 
 def half_leg(steps: int, prec_name: str, G: int = 12):
@@ -454,6 +516,7 @@ def legacy_leg(calls: int = 6):
     res = np.zeros_like(psi)
     lg.init_sse_su3dslash(LATT)
     out = {}
+    fl = 1320 * (int(np.prod(LATT)) // 2)
     try:
         lg.sse_su3dslash_wilson(u, psi, res, 1, 0)
         t0 = time.perf_counter()
@@ -461,7 +524,19 @@ def legacy_leg(calls: int = 6):
             lg.sse_su3dslash_wilson(u, psi, res, 1 if k % 4 < 2 else -1, k % 2)
         ms = (time.perf_counter() - t0) / calls * 1e3
         out["host_pointers_ms_per_call"] = round(ms, 2)
-        out["host_pointers_gflops"] = round(1320 * (int(np.prod(LATT)) // 2) / (ms * 1e-3) / 1e9, 1)
+        out["host_pointers_gflops"] = round(fl / (ms * 1e-3) / 1e9, 1)
+        # opt-in gauge reuse (sse_su3dslash_set_gauge_reuse): 1 = pointer + checksum,
+        # 2 = pointer only; the first call of each uploads, the timed ones reuse
+        for mode, key in ((1, "gauge_reuse_checksum"), (2, "gauge_reuse_pointer")):
+            assert lg.set_gauge_reuse(mode) == 0
+            lg.sse_su3dslash_wilson(u, psi, res, 1, 0)
+            t0 = time.perf_counter()
+            for k in range(calls):
+                lg.sse_su3dslash_wilson(u, psi, res, 1 if k % 4 < 2 else -1, k % 2)
+            m = (time.perf_counter() - t0) / calls * 1e3
+            out[key + "_ms_per_call"] = round(m, 2)
+            out[key + "_gflops"] = round(fl / (m * 1e-3) / 1e9, 1)
+        assert lg.set_gauge_reuse(0) == 0
         assert lg.gauge_upload(u, 18) == 0
         lg.wilson_resident(psi, res, 1, 0)
         t0 = time.perf_counter()
@@ -469,11 +544,13 @@ def legacy_leg(calls: int = 6):
             lg.wilson_resident(psi, res, 1 if k % 4 < 2 else -1, k % 2)
         ms2 = (time.perf_counter() - t0) / calls * 1e3
         out["resident_gauge_ms_per_call"] = round(ms2, 2)
-        out["resident_gauge_gflops"] = round(1320 * (int(np.prod(LATT)) // 2) / (ms2 * 1e-3) / 1e9, 1)
+        out["resident_gauge_gflops"] = round(fl / (ms2 * 1e-3) / 1e9, 1)
     finally:
         lg.free_sse_su3dslash()
     log(f"[legacy] host-pointer API {out['host_pointers_ms_per_call']} ms/call "
-        f"({out['host_pointers_gflops']} GFLOP/s, PCIe-inclusive); resident gauge "
+        f"({out['host_pointers_gflops']} GFLOP/s, PCIe-inclusive); gauge reuse "
+        f"{out['gauge_reuse_checksum_ms_per_call']} ms/call (checksum), "
+        f"{out['gauge_reuse_pointer_ms_per_call']} ms/call (pointer); resident gauge "
         f"{out['resident_gauge_ms_per_call']} ms/call ({out['resident_gauge_gflops']} GFLOP/s)")
     return out
 
@@ -483,7 +560,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0)
+    ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 3D / 12-real variants")
@@ -494,6 +572,9 @@ def main() -> int:
     ap.add_argument("--pmc-workload", action="store_true",
                     help="only a 2 GiB calibration copy and the headline kernel (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args.cpu_child, args.cpu_seconds)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the Dslash here runs on one GPU only "
@@ -541,8 +622,19 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_leg(args.cpu_seconds, threads)
+        host = host_cpu_info()
+        threads = args.cpu_threads or host["affinity_cpus"]
+        if host.get("cgroup_cpu_quota"):  # more threads than the CPU share only queue up
+            threads = max(1, min(threads, int(host["cgroup_cpu_quota"])))
+        c = cpu_leg(args.cpu_seconds, threads)
+        extra["cpu_dslash"] = c
+        a = c["f32_all_threads"]
+        cpu = {"value": a["gflops"], "unit": "GFLOP/s", "cores": threads, "kind": "port",
+               "sample": f"{a['calls']} calls of the host-CPU drop-in library (FP32 4D, 32x32x32x64, one "
+                         f"checkerboard each, alternating isign and cb) on {threads} threads of "
+                         f"{host.get('physical_cores')} physical cores ({host.get('model')}), median "
+                         f"{a['ms_per_call']} ms/call; 1 thread: {c['f32_1_thread']['ms_per_call']} ms/call"}
+        extra["gpu_vs_cpu_all_threads"] = round(head["gflops"] / a["gflops"], 1)
 
     gbs = head["gbs"]
     traffic = measured_traffic("dslash_wilson_f32_32c64_pmc.txt")

@@ -12,6 +12,8 @@ import os
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
 CORE = "libdslash_mi355x.so"
 LEGACY = {32: "libsse_dslash_f32.so", 64: "libsse_dslash_f64.so"}
+# the same drop-in API on the host CPU (OpenMP), for nodes without a GPU
+LEGACY_CPU = {32: "libsse_dslash_cpu_f32.so", 64: "libsse_dslash_cpu_f64.so"}
 
 F32, F64, F16, BF16, F64S = 0, 1, 2, 3, 4
 PREC_NAMES = {F32: "f32", F64: "f64", F16: "f16", BF16: "bf16", F64S: "f64_sloppy"}

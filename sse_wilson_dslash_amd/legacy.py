@@ -1,6 +1,8 @@
 """Python mirror of the reference's drop-in C API (include/sse_dslash.h,
 include/sse_dslash_3d.h), bound by ctypes to libsse_dslash_f32.so /
-libsse_dslash_f64.so.
+libsse_dslash_f64.so (the MI355X libraries) or, with backend="cpu", to
+libsse_dslash_cpu_f32.so / libsse_dslash_cpu_f64.so (the same API computed on
+the host CPU with OpenMP; chosen explicitly, never as a fallback).
 
 Same names, argument meaning and error behaviour as the reference:
 ``init_sse_su3dslash(latt_size, getSiteCoords, getLinearSiteIndex, nodeNumber)``
@@ -64,13 +66,18 @@ def table_callbacks(latt, site_table: np.ndarray, nd: int = 4):
 class SSEDslash:
     """One precision of the drop-in library (precision = 32 or 64)."""
 
-    def __init__(self, precision: int = 64):
+    def __init__(self, precision: int = 64, backend: str = "gpu"):
         if precision not in (32, 64):
             raise ValueError("precision must be 32 or 64")
-        _lib.core()  # the legacy library resolves the core library from the same directory
+        if backend not in ("gpu", "cpu"):
+            raise ValueError("backend must be 'gpu' or 'cpu'")
+        self.backend = backend
+        if backend == "gpu":
+            _lib.core()  # the legacy library resolves the core library from the same directory
         self.precision = precision
         self.dtype = np.float32 if precision == 32 else np.float64
-        self.lib = C.CDLL(_lib.lib_path(_lib.LEGACY[precision]), mode=C.RTLD_LOCAL)
+        name = (_lib.LEGACY if backend == "gpu" else _lib.LEGACY_CPU)[precision]
+        self.lib = C.CDLL(_lib.lib_path(name), mode=C.RTLD_LOCAL)
         L = self.lib
         for name in ("init_sse_su3dslash", "init_sse_su3dslash_3d"):
             getattr(L, name).restype = None

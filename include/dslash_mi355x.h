@@ -248,6 +248,24 @@ int dslash_cg_apply_op(void* solver, void* d_out, const void* d_in, int sloppy);
 int dslash_cg_set_params(void* solver, const dslash_cg_params* params);
 int dslash_cg_destroy(void* solver);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): out = A in with A = Mhat^dagger Mhat, Mhat = 1 - kappa^2
+ * D_eo D_oe, on the even checkerboard -- what dslash_cg_apply_op computes with
+ * a solver's workspace, here with no solver.  d_in / d_out: even-cb fields in
+ * format prec (DSLASH_F32, F64, F16, BF16 or F64_SLOPPY); d_gauge: the bundles
+ * of target cb 0 then 1 in that format (dslash_pack_gauge).  The odd-cb and
+ * intermediate fields live in one fixed 64 MiB device array of this library
+ * (each GPU it runs on holds its own copy until the process ends), so Vh is at
+ * most ~174k sites in FP64 (e.g. 16^3 x 32, 24^4) and ~349k in FP32 (24^3 x
+ * 48); larger lattices return DSLASH_ERR_INVALID.  Calls no hipMalloc/hipFree.
+ * Asynchronous on `stream`.  All calls on one GPU share the array: calls that
+ * overlap in time on one GPU (different streams, or host threads) race; calls
+ * on one stream are ordered.  The first call on each device resolves the
+ * array's address (hipGetSymbolAddress); later calls may be captured. */
+int dslash_normal_op_noalloc(void* d_out, const void* d_in, const void* d_gauge, const int latt[4], double kappa,
+                             int prec, int gauge_reals, void* stream);
+
 /* ---- backward (gauge force) -------------------------------------------- */
 /* Link gradient of S = Re <chi, D psi>, D from source checkerboard cb to 1-cb
  * with the given isign (4D, FP64).  chi, psi: full-lattice F64 device fields

@@ -85,17 +85,7 @@ static void cg_free(CgSolver* s) {
 // With qnorm, the per-block partials of |q|^2 = |Mhat in|^2 = <in, A in> are written there.
 static int apply_normal(const CgSolver* s, void* out, const void* in, void* t_odd, void* q_even, int prec, int G,
                         const void* g, size_t gcb, double* qnorm = nullptr) {
-  const double k2 = -s->prm.kappa * s->prm.kappa;
-  const char* g0 = (const char*)g;
-  const char* g1 = g0 + gcb;
-  hipStream_t st = s->stream;
-  // q = in - kappa^2 D_eo D_oe in
-  if (int e = apply_cb_ex(t_odd, in, g1, s->latt, 4, prec, G, +1, 1, nullptr, 0.0, st)) return e;
-  if (int e = apply_cb_ex(q_even, t_odd, g0, s->latt, 4, prec, G, +1, 0, in, k2, st, qnorm)) return e;
-  // out = q - kappa^2 (D^dag)_eo (D^dag)_oe q
-  if (int e = apply_cb_ex(t_odd, q_even, g1, s->latt, 4, prec, G, -1, 1, nullptr, 0.0, st)) return e;
-  if (int e = apply_cb_ex(out, t_odd, g0, s->latt, 4, prec, G, -1, 0, q_even, k2, st)) return e;
-  return 0;
+  return apply_normal_op(out, in, t_odd, q_even, g, gcb, s->latt, s->prm.kappa, prec, G, s->stream, qnorm);
 }
 
 // One inner iteration: Ap = A p ; alpha = rr / <p,Ap> ; x += alpha p ; r -= alpha Ap ; RRNEW = |r|^2

@@ -28,4 +28,9 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
                 int isign, int tcb, const void* xp_cb, double a, hipStream_t s, double* norm_partial = nullptr);
 int apply_blocks(const int latt[4], int nd, int prec, int G);
 
+// out = Mhat^dagger Mhat in on the even checkerboard (normal_op.hip); t_odd and
+// q_even are work fields of format prec, g the bundles of target cb 0 and 1.
+int apply_normal_op(void* out, const void* in, void* t_odd, void* q_even, const void* g, size_t gcb,
+                    const int latt[4], double kappa, int prec, int G, hipStream_t st, double* qnorm);
+
 }  // namespace wd

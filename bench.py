@@ -157,14 +157,23 @@ def cpu_child(prec: int, seconds: float, nd: int = 4) -> None:
     """Runs in a child process (OMP_* set by the parent): times the host-CPU
     library (libsse_dslash_cpu_f{32,64}.so, the drop-in API computed with
     OpenMP) at 32^3 x 64 and prints one JSON line."""
+    # no torch in this process: its OpenMP runtime would set its own thread count
+    import ctypes as C
     import numpy as np
-    from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
+    from sse_wilson_dslash_amd import _lib
     from sse_wilson_dslash_amd.legacy import SSEDslash
 
     lg = SSEDslash(prec, backend="cpu")
-    u = gen_gauge_host(LATT, nd, seed=1, dtype=lg.dtype)
-    psi = gen_spinor_host(LATT, nd, seed=1, dtype=lg.dtype)
+    core = _lib.core()
+    V = int(np.prod(LATT))
+    pc = _lib.F32 if prec == 32 else _lib.F64
+    u = np.empty((V, 4, 3, 3, 2), dtype=lg.dtype)
+    psi = np.empty((V, 4, 3, 2), dtype=lg.dtype)
+    _lib.check(core.dslash_gen_gauge_host(u.ctypes.data, pc, _lib.latt_arg(LATT), nd, 1))
+    _lib.check(core.dslash_gen_spinor_host(psi.ctypes.data, pc, _lib.latt_arg(LATT), nd, 1, 0))
     res = np.zeros_like(psi)
+    lg.lib.sse_su3dslash_cpu_threads.restype = C.c_int
+    threads = lg.lib.sse_su3dslash_cpu_threads()
     init, apply = ((lg.init_sse_su3dslash, lg.sse_su3dslash_wilson) if nd == 4 else
                    (lg.init_sse_su3dslash_3d, lg.sse_su3dslash_wilson_3d))
     init(LATT)
@@ -182,7 +191,7 @@ def cpu_child(prec: int, seconds: float, nd: int = 4) -> None:
     med = times[len(times) // 2]
     vh = int(np.prod(LATT)) // 2
     fl = (1320 if nd == 4 else 984) * vh
-    print(json.dumps({"ms": med * 1e3, "gflops": fl / med / 1e9, "calls": k,
+    print(json.dumps({"ms": med * 1e3, "gflops": fl / med / 1e9, "calls": k, "threads": threads,
                       "wall_s": time.perf_counter() - t_start}), flush=True)
 
 
@@ -203,6 +212,8 @@ def cpu_leg(seconds: float, threads: int):
             if r.returncode != 0:
                 raise RuntimeError(f"CPU leg failed: {r.stderr[-2000:]}")
             d = json.loads(r.stdout.strip().splitlines()[-1])
+            if d["threads"] != n:
+                raise RuntimeError(f"CPU leg asked for {n} OpenMP threads, the library runs {d['threads']}")
             key = f"f{prec}_{'all' if n > 1 else '1'}_thread{'s' if n > 1 else ''}"
             out[key] = {"ms_per_call": round(d["ms"], 2), "gflops": round(d["gflops"], 2), "calls": d["calls"],
                         "threads": n}
@@ -360,12 +371,47 @@ def copy_leg(gib: float = 2.0, reps: int = 20):
     return {"gbs": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes": 2 * n * 4}
 
 
+def ceiling_leg(reps: int = 200):
+    """The streaming ceiling of the headline kernel's traffic (dslash_time_ceiling:
+    the FP32 operator's 42 read / 6 written planes per site, links non-temporal,
+    no gathers), over block sizes with and without the XCD block order; the best
+    is the practical floor for moving 768 B/site on this GPU."""
+    import ctypes as C
+    import torch
+    from sse_wilson_dslash_amd import F32
+    from sse_wilson_dslash_amd._lib import check, core, latt_arg, ptr
+    from sse_wilson_dslash_amd.device import WilsonDslash, current_stream
+
+    op = WilsonDslash(LATT, 4, F32, 18)
+    vh = op.vh
+    a = torch.ones(96 * vh, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    links = torch.ones(576 * vh, dtype=torch.uint8, device="cuda")
+    lib = core()
+    res = {}
+    ms = C.c_float()
+    for block, remap in ((64, 1), (128, 1), (256, 1), (64, 0), (256, 0)):
+        check(lib.dslash_time_ceiling(ptr(b), ptr(a), ptr(links), latt_arg(LATT), block, remap, 20, current_stream(),
+                                      C.byref(ms)))
+        check(lib.dslash_time_ceiling(ptr(b), ptr(a), ptr(links), latt_arg(LATT), block, remap, reps,
+                                      current_stream(), C.byref(ms)))
+        res[f"{block}{'_xcd' if remap else ''}"] = round(ms.value * 1e3, 2)
+    best = min(res.values())
+    gbs = 768 * vh / (best * 1e-6) / 1e9
+    log(f"[ceiling] 42 read + 6 written planes/site at 32^3x64: best {best:.1f} us = {gbs:.0f} GB/s "
+        f"({gbs / PEAK_HBM_GBS:.1%} of peak); all shapes (us): {res}")
+    del a, b, links, op
+    torch.cuda.empty_cache()
+    return {"us_per_launch": best, "gbs": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4),
+            "bytes_per_launch": 768 * vh, "shapes_us": res}
+
+
 def cg_leg(latt=(48, 48, 48, 96), kappa=0.12, tol=1e-10):
     """Mixed-precision (F16 spinors + norms, 12-real F16 links, FP64 reliable
     updates) vs plain FP64 even-odd CG on Mhat^dag Mhat, same Gaussian source,
     time to FP64 true residual tol."""
     import torch
-    from sse_wilson_dslash_amd import F16, F64
+    from sse_wilson_dslash_amd import BF16, F16, F64
     from sse_wilson_dslash_amd._lib import check, latt_arg, ptr
     from sse_wilson_dslash_amd.device import WilsonDslash, current_stream
     from sse_wilson_dslash_amd.solver import WilsonCG
@@ -381,7 +427,8 @@ def cg_leg(latt=(48, 48, 48, 96), kappa=0.12, tol=1e-10):
     # plain FP64 CG: no reliable updates except the one confirming convergence (delta = 0);
     # mixed: reliable update when |r| has dropped by delta = 3e-3 since the last one.
     # Each pair shares the FP64 operator's links (18 reals, or 12 reals for SU(3)).
-    runs = (("fp64", F64, 18, 0.0, 18), ("mixed_f16", F16, 12, 3e-3, 18),
+    # BF16 keeps 8 significant bits, so its reliable updates come sooner (delta = 1e-2)
+    runs = (("fp64", F64, 18, 0.0, 18), ("mixed_f16", F16, 12, 3e-3, 18), ("mixed_bf16", BF16, 12, 1e-2, 18),
             ("fp64_g12", F64, 12, 0.0, 12), ("mixed_f16_g12", F16, 12, 3e-3, 12))
     for name, prec, G, delta, Gh in runs:
         cg = WilsonCG(latt, kappa=kappa, tol=tol, prec_sloppy=prec, gauge_reals_sloppy=G, delta=delta,
@@ -395,16 +442,18 @@ def cg_leg(latt=(48, 48, 48, 96), kappa=0.12, tol=1e-10):
         out[name] = {"ms": round(r["seconds"] * 1e3, 2), "iterations": r["iterations"],
                      "reliable_updates": r["reliable_updates"], "true_residual": r["true_residual"],
                      "converged": r["converged"], "delta": delta,
-                     "sloppy": {F64: "f64", F16: "f16"}[prec] + f" spinors, {G}-real links",
+                     "sloppy": {F64: "f64", F16: "f16", BF16: "bf16"}[prec] + f" spinors, {G}-real links",
                      "fp64_operator_links": Gh}
         cg.close()
         del x
         torch.cuda.empty_cache()
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed_f16"]["ms"], 3)
+    out["speedup_mixed_bf16_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed_bf16"]["ms"], 3)
     out["speedup_mixed_vs_fp64_g12_links"] = round(out["fp64_g12"]["ms"] / out["mixed_f16_g12"]["ms"], 3)
     out["config"] = {"lattice": list(latt), "kappa": kappa, "tol": tol, "operator": "Mhat^dag Mhat, even-odd",
                      "source": "Gaussian, even checkerboard, seed 1"}
-    log(f"[cg] mixed vs FP64 speedup: {out['speedup_mixed_vs_fp64']:.2f}x (18-real FP64 links), "
+    log(f"[cg] mixed vs FP64 speedup: {out['speedup_mixed_vs_fp64']:.2f}x (18-real FP64 links; BF16 "
+        f"{out['speedup_mixed_bf16_vs_fp64']:.2f}x), "
         f"{out['speedup_mixed_vs_fp64_g12_links']:.2f}x (12-real FP64 links)")
     del op, d_u, b
     torch.cuda.empty_cache()
@@ -595,8 +644,9 @@ def main() -> int:
         gpu_leg(args.steps, args.warmup, "f32", 4, 18, check_err=False)
         return 0
     head = gpu_leg(args.steps, args.warmup, "f32", 4, 18)
-    extra = {"copy": copy_leg(),
+    extra = {"copy": copy_leg(), "ceiling": ceiling_leg(),
              "f32_4d": {"rel_err_vs_fp64": head.get("rel_err_vs_fp64"), "launch": head["launch"]}}
+    extra["f32_4d"]["frac_of_ceiling"] = round(extra["ceiling"]["us_per_launch"] / (head["ms"] * 1e3), 4)
     if not args.no_extra:
         n = max(200, args.steps // 4)
         for key, (p, nd, G) in {"f64_4d": ("f64", 4, 18), "f32_4d_g12": ("f32", 4, 12),

@@ -351,6 +351,17 @@ int dslash_time_apply(void* d_res_full, const void* d_psi_full, const void* d_ga
                       const int latt[4], int nd, int prec, int gauge_reals,
                       int n, int alternate, void* stream, float* ms);
 
+/* Streaming ceiling of the 4D FP32 operator's traffic: n launches of a kernel
+ * that, per target site, reads 6 planes of d_in and 36 planes of d_links
+ * (non-temporally) and writes 6 planes of d_out, all at the site's own index
+ * (no gathers), in blocks of block_threads (64..256) with (xcd_remap = 1) or
+ * without the XCD-contiguous block order; 768 bytes per site as the Dslash's
+ * compulsory traffic.  Buffers: d_in and d_out >= 96 * Vh bytes (an FP32
+ * checkerboard), d_links >= 576 * Vh bytes (an FP32 18-real bundle).  Returns
+ * the average milliseconds per launch (HIP events on `stream`). */
+int dslash_time_ceiling(void* d_out, const void* d_in, const void* d_links, const int latt[4], int block_threads,
+                        int xcd_remap, int n, void* stream, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

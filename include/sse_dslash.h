@@ -87,6 +87,10 @@ int sse_su3dslash_set_gauge_reuse(int mode);
  * (upload it again). */
 int sse_su3dslash_set_sloppy(int on);
 
+/* CPU build (libsse_dslash_cpu_f32/f64.so): the OpenMP threads a call uses
+ * (OMP_NUM_THREADS or the runtime's default); the GPU build returns 0. */
+int sse_su3dslash_cpu_threads(void);
+
 /* The reference's C gauge packer (lib/packer_nopad.c:29-45, declared in no
  * reference header): reorders [mu][cb][Vh] links into the packed
  * [cb][Vh][mu] order, without transposing.  volume = V/2. */

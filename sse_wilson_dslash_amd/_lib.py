@@ -52,6 +52,7 @@ PROTOTYPES = {
     "dslash_cg_set_params": (_i, [_vp, C.POINTER(CgParams)]),
     "dslash_cg_destroy": (_i, [_vp]),
     "dslash_normal_op_noalloc": (_i, [_vp, _vp, _vp, _ip, C.c_double, _i, _i, _vp]),
+    "dslash_time_ceiling": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _vp, _fp]),
     "dslash_last_error": (_i, []),
     "dslash_last_error_message": (C.c_char_p, []),
     "dslash_version": (_i, []),

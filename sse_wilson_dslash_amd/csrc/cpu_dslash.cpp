@@ -512,6 +512,8 @@ void sse_su3dslash_wilson(SSEREAL* u, SSEREAL* psi, SSEREAL* res, int isign, int
 }
 void sse_su3dslash_prepost_receives(void) {}
 int sse_su3dslash_last_error(void) { return g4.last_error; }
+// Extension of the CPU build only: the OpenMP threads a call will use.
+int sse_su3dslash_cpu_threads(void) { return omp_get_max_threads(); }
 
 void init_sse_su3dslash_3d(const int* latt_size, void (*getSiteCoords)(int coord[], int node, int linearsite),
                            int (*getLinearSiteIndex)(const int coord[]), int (*nodeNumber)(const int coord[])) {

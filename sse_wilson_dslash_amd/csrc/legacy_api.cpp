@@ -399,6 +399,8 @@ void sse_su3dslash_wilson(SSEREAL* u, SSEREAL* psi, SSEREAL* res, int isign, int
   (void)g4.wilson(u, psi, res, isign, cb);
 }
 void sse_su3dslash_prepost_receives(void) {}
+// The CPU build's thread query: no host threads compute here.
+int sse_su3dslash_cpu_threads(void) { return 0; }
 int sse_su3dslash_last_error(void) { return g4.last_error; }
 int sse_su3dslash_gauge_upload(SSEREAL* u, int gauge_reals) { return g4.gauge_upload(u, gauge_reals); }
 int sse_su3dslash_gauge_release(void) { return g4.gauge_release(); }

@@ -46,7 +46,7 @@ def test_cpu_library_exports():
              "sse_su3dslash_wilson_resident", "sse_su3dslash_wilson_device", "sse_su3dslash_set_sloppy",
              "sse_su3dslash_set_gauge_reuse", "sse_su3dslash_3d_gauge_upload", "sse_su3dslash_3d_gauge_release",
              "sse_su3dslash_wilson_3d_resident", "sse_su3dslash_wilson_3d_device", "sse_su3dslash_3d_set_sloppy",
-             "sse_su3dslash_3d_set_gauge_reuse"]
+             "sse_su3dslash_3d_set_gauge_reuse", "sse_su3dslash_cpu_threads"]
     import ctypes as C
     for prec in (32, 64):
         path = _lib.lib_path(_lib.LEGACY_CPU[prec])
@@ -55,6 +55,12 @@ def test_cpu_library_exports():
             assert hasattr(lib, n), (prec, n)
         deps = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
         assert "amdhip" not in deps and "dslash_mi355x" not in deps, deps
+    # the thread query follows OMP_NUM_THREADS in a fresh process
+    code = (f"import ctypes; L = ctypes.CDLL({_lib.lib_path(_lib.LEGACY_CPU[32])!r}); "
+            "print(L.sse_su3dslash_cpu_threads())")
+    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, OMP_NUM_THREADS="3"),
+                       capture_output=True, text=True, timeout=60)
+    assert r.stdout.strip() == "3", r.stdout + r.stderr
 
 
 @pytest.mark.parametrize("prec", [32, 64])

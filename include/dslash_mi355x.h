@@ -89,6 +89,15 @@ int dslash_pack_spinor(void* d_cb, int cb, const void* d_aos, int prec_in,
 int dslash_unpack_spinor(void* d_aos, int cb, const void* d_cb, int prec_cb,
                          const int latt[4], int nd, int prec_out,
                          const int* d_site_table, int compact, void* stream);
+/* The same for the sites [site_begin, site_end) of the checkerboard only (site
+ * index within the checkerboard, canonical order; other sites untouched), so a
+ * caller can overlap transfers and work slab by slab. */
+int dslash_pack_spinor_sites(void* d_cb, int cb, const void* d_aos, int prec_in, const int latt[4], int nd,
+                             int prec_out, const int* d_site_table, size_t site_begin, size_t site_end,
+                             void* stream);
+int dslash_unpack_spinor_sites(void* d_aos, int cb, const void* d_cb, int prec_cb, const int latt[4], int nd,
+                               int prec_out, const int* d_site_table, int compact, size_t site_begin,
+                               size_t site_end, void* stream);
 /* Precision conversion of one checkerboard (either direction, any formats). */
 int dslash_convert_spinor(void* d_out, int prec_out, const void* d_in, int prec_in,
                           const int latt[4], int nd, void* stream);
@@ -103,6 +112,16 @@ int dslash_convert_spinor(void* d_out, int prec_out, const void* d_in, int prec_
 int dslash_apply(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb,
                  const int latt[4], int nd, int prec, int gauge_reals,
                  int isign, int target_cb, void* stream);
+
+/* dslash_apply on the target sites [site_begin, site_end) of the checkerboard
+ * only (canonical order, one site per thread; other target sites are not
+ * written).  In the canonical orders the slowest coordinate (t in 4D, z in 3D)
+ * is outermost, so a range of whole slices needs only the source slices next
+ * to it: the drop-in API uses this to compute slab k while slab k + 1 is
+ * still arriving over PCIe. */
+int dslash_apply_sites(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb, const int latt[4], int nd,
+                       int prec, int gauge_reals, int isign, int target_cb, size_t site_begin, size_t site_end,
+                       void* stream);
 
 /* 4D only: res_cb = x_cb + a * D psi_cb in one pass (x_cb may alias res_cb).
  * With a = -kappa this is the Wilson matrix M = 1 - kappa D between

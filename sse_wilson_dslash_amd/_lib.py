@@ -53,6 +53,9 @@ PROTOTYPES = {
     "dslash_cg_destroy": (_i, [_vp]),
     "dslash_normal_op_noalloc": (_i, [_vp, _vp, _vp, _ip, C.c_double, _i, _i, _vp]),
     "dslash_time_ceiling": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _vp, _fp]),
+    "dslash_pack_spinor_sites": (_i, [_vp, _i, _vp, _i, _ip, _i, _i, _vp, _sz, _sz, _vp]),
+    "dslash_unpack_spinor_sites": (_i, [_vp, _i, _vp, _i, _ip, _i, _i, _vp, _i, _sz, _sz, _vp]),
+    "dslash_apply_sites": (_i, [_vp, _vp, _vp, _ip, _i, _i, _i, _i, _i, _sz, _sz, _vp]),
     "dslash_last_error": (_i, []),
     "dslash_last_error_message": (C.c_char_p, []),
     "dslash_version": (_i, []),

@@ -117,6 +117,8 @@ struct ApplyArgs {
   int march;        // slices per thread of k_dslash_march (1 = k_dslash)
   uint32_t slice;   // sites per slice of the slowest coordinate (t in 4D, z in 3D)
   FastDiv dslice;
+  uint32_t site0 = 0;       // k_dslash: target sites [site0, site_end) of the checkerboard
+  uint32_t site_end = 0;
 };
 
 template <int ND, int P, int G, int IS, int MU>
@@ -154,8 +156,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWav
 k_dslash(ApplyArgs a) {
   typedef typename Fmt<P>::compute T;
   uint32_t b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  uint32_t idx = b * blockDim.x + threadIdx.x;
-  const bool live = idx < a.g.vh;
+  uint32_t idx = a.site0 + b * blockDim.x + threadIdx.x;
+  const bool live = idx < a.site_end;
   if constexpr (EPI != EPI_XPAY_NORM) {
     if (!live) return;
   }
@@ -284,7 +286,7 @@ k_dslash_march(ApplyArgs a) {
 
 template <int ND, int P, int G, int IS>
 static void launch_dslash_t(const ApplyArgs& a, hipStream_t s) {
-  uint32_t blocks = (a.g.vh + a.block - 1) / a.block;
+  uint32_t blocks = (a.site_end - a.site0 + a.block - 1) / a.block;
   if (!a.xp && a.march > 1) {
     const uint32_t chunks = (uint32_t)(a.g.L[ND - 1] + a.march - 1) / (uint32_t)a.march;
     const uint32_t mblocks = (a.slice * chunks + a.block - 1) / a.block;
@@ -390,10 +392,55 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
   a.block = lc.block;
   a.slice = a.g.vh / (uint32_t)latt[nd - 1];
   a.dslice = make_fastdiv(a.slice);
+  a.site0 = 0;
+  a.site_end = a.g.vh;
   a.march = effective_march(latt, nd, prec, lc.block, lc.march);
   if (a.march == 16)  // LDS-staged box tiles, every epilogue form
     return box_apply_single(res_cb, psi_cb, gauge_cb, latt, prec, G, isign, tcb, lc.block,
                             BoxEpi{a.xp, xa, norm_partial}, s);
+  if (nd == 4)
+    launch_dslash_nd<4>(a, prec, G, isign, s);
+  else
+    launch_dslash_nd<3>(a, prec, G, isign, s);
+  WD_LAUNCHED();
+  return 0;
+}
+
+// res = D psi on the target sites [site0, site1) of the checkerboard only, one
+// site per thread (the other target sites are not written).
+static int apply_cb_sites(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int nd, int prec,
+                          int G, int isign, int tcb, uint64_t site0, uint64_t site1, hipStream_t s) {
+  if (int e = check_lattice(latt, nd)) return e;
+  if (int e = check_prec(prec)) return e;
+  WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12 (got %d)", G);
+  WD_REQUIRE(isign == 1 || isign == -1, "isign must be +1 or -1 (got %d)", isign);
+  WD_REQUIRE(tcb == 0 || tcb == 1, "checkerboard must be 0 or 1 (got %d)", tcb);
+  WD_REQUIRE(res_cb && psi_cb && gauge_cb, "NULL field pointer");
+  WD_REQUIRE(res_cb != psi_cb, "res and psi must be different checkerboard buffers");
+  ApplyArgs a;
+  a.g = make_geom(latt, nd);
+  WD_REQUIRE(site0 <= site1 && site1 <= a.g.vh, "site range [%llu, %llu) outside the %u sites of a checkerboard",
+             (unsigned long long)site0, (unsigned long long)site1, a.g.vh);
+  if (site0 == site1) return 0;
+  const size_t planes = (size_t)spinor_vec(prec) * 16 * a.g.vh;
+  a.psi = (const v4u*)psi_cb;
+  a.psi_norm = (const float*)((const char*)psi_cb + planes);
+  a.res = (v4u*)res_cb;
+  a.res_norm = (float*)((char*)res_cb + planes);
+  a.gauge = (const v4u*)gauge_cb;
+  a.xp = nullptr;
+  a.xp_norm = nullptr;
+  a.a = 0.0;
+  a.norm_partial = nullptr;
+  a.tcb = tcb;
+  const LaunchCfg lc = launch_cfg(nd, prec, G);
+  a.remap = lc.remap;
+  a.block = lc.block;
+  a.slice = a.g.vh / (uint32_t)latt[nd - 1];
+  a.dslice = make_fastdiv(a.slice);
+  a.site0 = (uint32_t)site0;
+  a.site_end = (uint32_t)site1;
+  a.march = 1;
   if (nd == 4)
     launch_dslash_nd<4>(a, prec, G, isign, s);
   else
@@ -452,10 +499,10 @@ __global__ void k_pack_gauge(v4u* __restrict__ out, const typename Fmt<PIN>::rea
 template <int PIN, int POUT>
 __global__ void k_pack_spinor(v4u* __restrict__ out, float* __restrict__ out_norm,
                               const typename Fmt<PIN>::real* __restrict__ aos, const int* __restrict__ table,
-                              uint32_t vh, int cb) {
+                              uint32_t vh, int cb, uint32_t i0, uint32_t i1) {
   typedef typename Fmt<POUT>::compute T;
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= vh) return;
+  uint32_t idx = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= i1) return;
   uint32_t mx = (uint32_t)cb * vh + idx;
   size_t cx = table ? (size_t)table[mx] : mx;
   const typename Fmt<PIN>::real* p = aos + cx * 24;
@@ -473,9 +520,9 @@ __global__ void k_pack_spinor(v4u* __restrict__ out, float* __restrict__ out_nor
 template <int PCB, int POUT>
 __global__ void k_unpack_spinor(typename Fmt<POUT>::real* __restrict__ aos, const v4u* __restrict__ in,
                                 const float* __restrict__ in_norm, const int* __restrict__ table, uint32_t vh,
-                                int cb, int compact) {
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= vh) return;
+                                int cb, int compact, uint32_t i0, uint32_t i1) {
+  uint32_t idx = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= i1) return;
   Spinor<typename Fmt<PCB>::compute> s = load_spinor<PCB>(in, in_norm, idx, vh);
   uint32_t mx = (uint32_t)cb * vh + idx;
   size_t dst = compact ? idx : (table ? (size_t)table[mx] : mx);
@@ -734,8 +781,9 @@ int dslash_pack_gauge(void* d_bundle, int target_cb, const void* d_u_packed, int
   return 0;
 }
 
-int dslash_pack_spinor(void* d_cb, int cb, const void* d_aos, int prec_in, const int latt[4], int nd, int prec_out,
-                       const int* d_site_table, void* stream) {
+int dslash_pack_spinor_sites(void* d_cb, int cb, const void* d_aos, int prec_in, const int latt[4], int nd,
+                             int prec_out, const int* d_site_table, size_t site_begin, size_t site_end,
+                             void* stream) {
   clear_error();
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec_out)) return e;
@@ -744,14 +792,18 @@ int dslash_pack_spinor(void* d_cb, int cb, const void* d_aos, int prec_in, const
   WD_REQUIRE(cb == 0 || cb == 1, "cb must be 0 or 1");
   WD_REQUIRE(d_cb && d_aos, "NULL pointer");
   Geom g = make_geom(latt, nd);
+  WD_REQUIRE(site_begin <= site_end && site_end <= g.vh, "site range [%zu, %zu) outside the %u sites of a checkerboard",
+             site_begin, site_end, g.vh);
+  if (site_begin == site_end) return 0;
+  const uint32_t i0 = (uint32_t)site_begin, i1 = (uint32_t)site_end;
   size_t planes = (size_t)spinor_vec(prec_out) * 16 * g.vh;
   v4u* out = (v4u*)d_cb;
   float* nrm = (float*)((char*)d_cb + planes);
-  dim3 grid((g.vh + 255) / 256), blk(256);
+  dim3 grid((i1 - i0 + 255) / 256), blk(256);
   hipStream_t s = (hipStream_t)stream;
 #define PACK_S(PI, PO)                                                                                        \
   hipLaunchKernelGGL((k_pack_spinor<PI, PO>), grid, blk, 0, s, out, nrm, (const typename Fmt<PI>::real*)d_aos, \
-                     d_site_table, g.vh, cb)
+                     d_site_table, g.vh, cb, i0, i1)
 #define PACK_S_PO(PI)                          \
   switch (prec_out) {                          \
     case F32: PACK_S(PI, F32); break;          \
@@ -766,8 +818,16 @@ int dslash_pack_spinor(void* d_cb, int cb, const void* d_aos, int prec_in, const
   return 0;
 }
 
-int dslash_unpack_spinor(void* d_aos, int cb, const void* d_cb, int prec_cb, const int latt[4], int nd,
-                         int prec_out, const int* d_site_table, int compact, void* stream) {
+int dslash_pack_spinor(void* d_cb, int cb, const void* d_aos, int prec_in, const int latt[4], int nd, int prec_out,
+                       const int* d_site_table, void* stream) {
+  if (int e = check_lattice(latt, nd)) return e;
+  return dslash_pack_spinor_sites(d_cb, cb, d_aos, prec_in, latt, nd, prec_out, d_site_table, 0,
+                                  make_geom(latt, nd).vh, stream);
+}
+
+int dslash_unpack_spinor_sites(void* d_aos, int cb, const void* d_cb, int prec_cb, const int latt[4], int nd,
+                               int prec_out, const int* d_site_table, int compact, size_t site_begin,
+                               size_t site_end, void* stream) {
   clear_error();
   if (int e = check_lattice(latt, nd)) return e;
   if (int e = check_prec(prec_cb)) return e;
@@ -776,14 +836,18 @@ int dslash_unpack_spinor(void* d_aos, int cb, const void* d_cb, int prec_cb, con
   WD_REQUIRE(cb == 0 || cb == 1, "cb must be 0 or 1");
   WD_REQUIRE(d_cb && d_aos, "NULL pointer");
   Geom g = make_geom(latt, nd);
+  WD_REQUIRE(site_begin <= site_end && site_end <= g.vh, "site range [%zu, %zu) outside the %u sites of a checkerboard",
+             site_begin, site_end, g.vh);
+  if (site_begin == site_end) return 0;
+  const uint32_t i0 = (uint32_t)site_begin, i1 = (uint32_t)site_end;
   size_t planes = (size_t)spinor_vec(prec_cb) * 16 * g.vh;
   const v4u* in = (const v4u*)d_cb;
   const float* nrm = (const float*)((const char*)d_cb + planes);
-  dim3 grid((g.vh + 255) / 256), blk(256);
+  dim3 grid((i1 - i0 + 255) / 256), blk(256);
   hipStream_t s = (hipStream_t)stream;
 #define UNPACK_S(PC, PO)                                                                                      \
   hipLaunchKernelGGL((k_unpack_spinor<PC, PO>), grid, blk, 0, s, (typename Fmt<PO>::real*)d_aos, in, nrm, \
-                     d_site_table, g.vh, cb, compact)
+                     d_site_table, g.vh, cb, compact, i0, i1)
 #define UNPACK_S_PC(PO)                         \
   switch (prec_cb) {                            \
     case F32: UNPACK_S(F32, PO); break;         \
@@ -796,6 +860,21 @@ int dslash_unpack_spinor(void* d_aos, int cb, const void* d_cb, int prec_cb, con
 #undef UNPACK_S
   WD_LAUNCHED();
   return 0;
+}
+
+int dslash_unpack_spinor(void* d_aos, int cb, const void* d_cb, int prec_cb, const int latt[4], int nd,
+                         int prec_out, const int* d_site_table, int compact, void* stream) {
+  if (int e = check_lattice(latt, nd)) return e;
+  return dslash_unpack_spinor_sites(d_aos, cb, d_cb, prec_cb, latt, nd, prec_out, d_site_table, compact, 0,
+                                    make_geom(latt, nd).vh, stream);
+}
+
+int dslash_apply_sites(void* d_res_cb, const void* d_psi_cb, const void* d_gauge_cb, const int latt[4], int nd,
+                       int prec, int gauge_reals, int isign, int target_cb, size_t site_begin, size_t site_end,
+                       void* stream) {
+  clear_error();
+  return apply_cb_sites(d_res_cb, d_psi_cb, d_gauge_cb, latt, nd, prec, gauge_reals, isign, target_cb, site_begin,
+                        site_end, (hipStream_t)stream);
 }
 
 int dslash_convert_spinor(void* d_out, int prec_out, const void* d_in, int prec_in, const int latt[4], int nd,

@@ -16,6 +16,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "dslash_mi355x.h"
@@ -73,7 +77,17 @@ struct Legacy {
   const SSEREAL* reuse_ptr = nullptr;
   uint64_t reuse_sum = 0;
   std::vector<SSEREAL> h_stage;
-  wd::Stager stager;
+  wd::Stager stager;     // uploads (and everything that is not pipelined)
+  wd::Stager stager_dn;  // downloads of the slab pipeline, so both directions move at once
+  // Slab pipeline of the host-pointer call (canonical orders): the source
+  // checkerboard moves up in slabs of whole slices of the slowest coordinate
+  // on s_up while the compute stream works on the slabs already there and s_dn
+  // brings finished target slabs back.
+  static constexpr int MAX_SLABS = 32;
+  static constexpr int SLABS = 8;  // default slab count
+  hipStream_t s_up = nullptr, s_dn = nullptr;
+  hipEvent_t ev_up[MAX_SLABS] = {}, ev_done[MAX_SLABS] = {};
+  int nslab = 0;
 
   explicit Legacy(int nd_, const char* name_) : nd(nd_), name(name_) {}
 
@@ -173,7 +187,8 @@ struct Legacy {
     sb = dslash_spinor_cb_bytes(latt, nd, LEG_PREC);
     gb18 = dslash_gauge_cb_bytes(latt, nd, LEG_PREC, 18);
     hipError_t e;
-    if ((e = hipStreamCreate(&stream)) != hipSuccess ||
+    if ((e = hipStreamCreate(&stream)) != hipSuccess || (e = hipStreamCreate(&s_up)) != hipSuccess ||
+        (e = hipStreamCreate(&s_dn)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&dev_done, hipEventDisableTiming)) != hipSuccess ||
         (e = hipMalloc(&d_u_aos, V * 4 * 18 * RS)) != hipSuccess ||
         (e = hipMalloc(&d_psi_aos, V * 24 * RS)) != hipSuccess ||
@@ -194,6 +209,28 @@ struct Legacy {
       h_stage.resize(Vh * 24);
     }
     stager.start();
+    nslab = 0;
+    if (canonical) {
+      // slabs of whole slices, each at most one staging chunk: the first target
+      // slab waits for three source slabs and the last two downloads follow the
+      // last upload, which favours short slabs; each slab costs a few host
+      // synchronisations and launches, which favours few.  SSE_DSLASH_SLABS
+      // overrides the count.
+      const size_t L = (size_t)latt[nd - 1], slice_bytes = Vh / L * 24 * RS;
+      size_t S = std::max<size_t>(SLABS, (Vh * 24 * RS + wd::Stager::CHUNK - 1) / wd::Stager::CHUNK);
+      if (const char* env = getenv("SSE_DSLASH_SLABS")) S = (size_t)std::max(0, atoi(env));
+      S = std::min<size_t>(S, std::min<size_t>(L, MAX_SLABS));
+      if (S >= 3 && (L + S - 1) / S * slice_bytes <= wd::Stager::CHUNK) nslab = (int)S;
+    }
+    if (nslab) {
+      for (int k = 0; k < nslab; ++k)
+        if ((e = hipEventCreateWithFlags(&ev_up[k], hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&ev_done[k], hipEventDisableTiming)) != hipSuccess) {
+          fprintf(stderr, "%s: event creation failed: %s\n", name, hipGetErrorString(e));
+          exit(1);
+        }
+      stager_dn.start();
+    }
     resident_G = 0;
     dev_pending = false;
     reuse_valid = false;
@@ -207,14 +244,25 @@ struct Legacy {
     if (stream) (void)hipStreamSynchronize(stream);
     if (dev_pending) (void)hipEventSynchronize(dev_done);
     dev_pending = false;
+    if (s_up) (void)hipStreamSynchronize(s_up);
+    if (s_dn) (void)hipStreamSynchronize(s_dn);
     stager.finish();
+    stager_dn.finish();
+    for (int k = 0; k < MAX_SLABS; ++k) {
+      if (ev_up[k]) (void)hipEventDestroy(ev_up[k]);
+      if (ev_done[k]) (void)hipEventDestroy(ev_done[k]);
+      ev_up[k] = ev_done[k] = nullptr;
+    }
+    nslab = 0;
     void* ptrs[] = {d_u_aos, d_psi_aos, d_res_aos, d_psi_cb, d_res_cb, d_bundle, d_bundle_call, d_site_table};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     d_u_aos = d_psi_aos = d_res_aos = d_psi_cb = d_res_cb = d_bundle = d_bundle_call = nullptr;
     d_site_table = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
-    stream = nullptr;
+    if (s_up) (void)hipStreamDestroy(s_up);
+    if (s_dn) (void)hipStreamDestroy(s_dn);
+    stream = s_up = s_dn = nullptr;
     if (dev_done) (void)hipEventDestroy(dev_done);
     dev_done = nullptr;
     site_table.clear();
@@ -254,8 +302,109 @@ struct Legacy {
     return 0;
   }
 
+  // First site (within a checkerboard) of slab k of the slab pipeline.
+  size_t slab_begin(int k) const {
+    const size_t L = (size_t)latt[nd - 1];
+    return L * (size_t)k / (size_t)nslab * (Vh / L);
+  }
+
+  // The host-pointer apply as a pipeline over slabs of whole slices of the
+  // slowest coordinate (t in 4D, z in 3D; canonical orders).  Target slab k
+  // needs source slabs k - 1, k, k + 1 (periodic), so the source slabs go up
+  // in the order S-1, 0, 1, ..., S-2 on s_up (a host thread, through the
+  // upload staging buffers), the compute stream packs each source slab once it
+  // has arrived and computes target slab k as soon as its three source slabs
+  // are in, and s_dn brings target slab k back (its own staging buffers)
+  // while slab k + 1 is computed and later source slabs still go up.
+  int apply_host_slabs(SSEREAL* psi, SSEREAL* res, int isign, int cb, const void* bundle_t, int G) {
+    const int tcb = 1 - cb, S = nslab;
+    const size_t site_bytes = 24 * RS;
+    std::mutex m;
+    std::condition_variable cv;
+    int uploaded = 0;  // slabs whose upload event is recorded
+    hipError_t up_err = hipSuccess;
+    std::thread up([&] {
+      for (int step = 0; step < S; ++step) {
+        const int j = step == 0 ? S - 1 : step - 1;
+        const size_t a = slab_begin(j), b = slab_begin(j + 1);
+        const size_t off = ((size_t)cb * Vh + a) * 24;
+        hipError_t e = stager.h2d((SSEREAL*)d_psi_aos + off, psi + off, (b - a) * site_bytes, s_up);
+        if (e == hipSuccess) e = hipEventRecord(ev_up[j], s_up);
+        std::lock_guard<std::mutex> lk(m);
+        if (e != hipSuccess) {
+          up_err = e;
+          uploaded = S;  // no further progress: let the consumer stop
+        } else {
+          uploaded = step + 1;
+        }
+        cv.notify_one();
+        if (e != hipSuccess) return;
+      }
+    });
+    auto step_of = [S](int j) { return j == S - 1 ? 0 : j + 1; };
+    std::vector<char> packed(S, 0);
+    int rc = 0;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < S && !rc; ++k) {
+      const int nb[3] = {(k + S - 1) % S, k, (k + 1) % S};
+      int need = 0;
+      for (int j : nb) need = std::max(need, step_of(j) + 1);
+      {
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return uploaded >= need; });
+        if (up_err != hipSuccess) break;
+      }
+      for (int j : nb) {
+        if (packed[j]) continue;
+        if ((e = hipStreamWaitEvent(stream, ev_up[j], 0)) != hipSuccess) break;
+        if (dslash_pack_spinor_sites(d_psi_cb, cb, d_psi_aos, LEG_PREC, latt, nd, prec, nullptr, slab_begin(j),
+                                     slab_begin(j + 1), stream)) {
+          rc = fail(dslash_last_error(), "spinor pack");
+          break;
+        }
+        packed[j] = 1;
+      }
+      if (e != hipSuccess || rc) break;
+      const size_t a = slab_begin(k), b = slab_begin(k + 1);
+      if (dslash_apply_sites(d_res_cb, d_psi_cb, bundle_t, latt, nd, prec, G, isign, tcb, a, b, stream)) {
+        rc = fail(dslash_last_error(), "dslash");
+        break;
+      }
+      if (dslash_unpack_spinor_sites(d_res_aos, tcb, d_res_cb, prec, latt, nd, LEG_PREC, nullptr, 0, a, b, stream)) {
+        rc = fail(dslash_last_error(), "spinor unpack");
+        break;
+      }
+      const size_t off = ((size_t)tcb * Vh + a) * 24;
+      if ((e = hipEventRecord(ev_done[k], stream)) != hipSuccess ||
+          (e = hipStreamWaitEvent(s_dn, ev_done[k], 0)) != hipSuccess ||
+          (e = stager_dn.d2h_issue(k & 1, (const SSEREAL*)d_res_aos + off, (b - a) * site_bytes, s_dn)) !=
+              hipSuccess)
+        break;
+      if (k > 0) {
+        const size_t a0 = slab_begin(k - 1), off0 = ((size_t)tcb * Vh + a0) * 24;
+        if ((e = stager_dn.d2h_finish(res + off0, (k - 1) & 1, (a - a0) * site_bytes)) != hipSuccess) break;
+      }
+    }
+    if (!rc && e == hipSuccess && up_err == hipSuccess) {
+      const size_t a0 = slab_begin(S - 1), off0 = ((size_t)tcb * Vh + a0) * 24;
+      e = stager_dn.d2h_finish(res + off0, (S - 1) & 1, (Vh - a0) * site_bytes);
+    }
+    up.join();
+    // leave no stream work behind that could still touch the caller's arrays
+    (void)hipStreamSynchronize(s_up);
+    (void)hipStreamSynchronize(s_dn);
+    (void)hipStreamSynchronize(stream);
+    if (rc) return rc;
+    if (up_err != hipSuccess) return hip_fail(up_err, "spinor upload");
+    if (e != hipSuccess) return hip_fail(e, "slab pipeline");
+    last_error = 0;
+    return 0;
+  }
+
   // Shared tail of the host-pointer paths: psi's source cb in, res target cb out.
   int apply_host(SSEREAL* psi, SSEREAL* res, int isign, int cb, const void* bundle_t, int G) {
+    if (nslab && stager.pipelined() && stager_dn.pipelined())
+      return apply_host_slabs(psi, res, isign, cb, bundle_t, G);
     const int tcb = 1 - cb;
     hipError_t e;
     if (canonical)
@@ -325,7 +474,17 @@ struct Legacy {
   // pointer than last time, or (mode 1) its checksum changed.
   int wilson_reuse(SSEREAL* u, SSEREAL* psi, SSEREAL* res, int isign, int cb) {
     const size_t ub = V * 4 * 18 * RS;
-    const uint64_t sum = reuse_mode == 1 ? stager.checksum(u, ub) : 0;
+    uint64_t sum = 0;
+    if (reuse_mode == 1 && nslab) {  // both staging pools hash half of the array each
+      const size_t h1 = ub / 2;
+      uint64_t s2 = 0;
+      std::thread t2([&] { s2 = stager_dn.checksum((const char*)u + h1, ub - h1); });
+      const uint64_t s1 = stager.checksum(u, h1);
+      t2.join();
+      sum = (s1 * 0x9E3779B97F4A7C15ull) ^ s2;
+    } else if (reuse_mode == 1) {
+      sum = stager.checksum(u, ub);
+    }
     if (!(reuse_valid && resident_G == 18 && u == reuse_ptr && (reuse_mode == 2 || sum == reuse_sum))) {
       reuse_valid = false;
       resident_G = 0;

@@ -24,6 +24,7 @@ namespace wd {
 class Stager {
  public:
   static constexpr size_t CHUNK = (size_t)32 << 20;
+  static constexpr size_t SMALL = (size_t)1 << 20;  // smaller uploads go straight to hipMemcpyAsync
 
   Stager() = default;
   Stager(const Stager&) = delete;
@@ -60,15 +61,19 @@ class Stager {
     if (pinned) (void)hipHostFree(pinned);
     pinned = nullptr;
     busy[0] = busy[1] = false;
+    next = 0;
   }
 
   // Enqueues the upload on s; returns once every byte of h has been read (the
-  // last DMAs may still be running from the staging buffers).
+  // last DMAs may still be running from the staging buffers).  Chunks alternate
+  // between the two buffers across calls too, so the host copy of one call's
+  // first chunk overlaps the DMA of the previous call's last one.
   hipError_t h2d(void* d, const void* h, size_t n, hipStream_t s) {
-    if (!usable() || n < 2 * CHUNK) return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s);
-    for (size_t i = 0, off = 0; off < n; ++i, off += CHUNK) {
+    if (!usable() || n < SMALL) return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s);
+    for (size_t off = 0; off < n; off += CHUNK) {
       const size_t len = std::min(CHUNK, n - off);
-      const int b = (int)(i & 1);
+      const int b = next;
+      next ^= 1;
       if (hipError_t e = reuse(b)) return e;
       copy((char*)pinned + b * CHUNK, (const char*)h + off, len);
       if (hipError_t e = hipMemcpyAsync((char*)d + off, pinned + b * CHUNK, len, hipMemcpyHostToDevice, s)) return e;
@@ -89,6 +94,26 @@ class Stager {
     for (int k = 0; k < job_parts; ++k) r = (r ^ part_sum[k]) * 0x9E3779B97F4A7C15ull, r ^= r >> 31;
     return r;
   }
+
+  // Split download for pipelines: d2h_issue starts the DMA of n <= CHUNK bytes
+  // into staging buffer `slot` (0 or 1) after the work already on s;
+  // d2h_finish waits for it and copies the bytes to h.  A slot is reused only
+  // after its finish.
+  hipError_t d2h_issue(int slot, const void* d, size_t n, hipStream_t s) {
+    if (!usable() || n > CHUNK || (slot & ~1)) return hipErrorInvalidValue;
+    if (hipError_t e = reuse(slot)) return e;
+    if (hipError_t e = hipMemcpyAsync(pinned + slot * CHUNK, d, n, hipMemcpyDeviceToHost, s)) return e;
+    if (hipError_t e = hipEventRecord(ev[slot], s)) return e;
+    busy[slot] = true;
+    return hipSuccess;
+  }
+  hipError_t d2h_finish(void* h, int slot, size_t n) {
+    if (!usable() || n > CHUNK || (slot & ~1)) return hipErrorInvalidValue;
+    if (hipError_t e = reuse(slot)) return e;
+    copy((char*)h, pinned + slot * CHUNK, n);
+    return hipSuccess;
+  }
+  bool pipelined() const { return usable(); }
 
   // Downloads after the work already on s; returns with h complete.
   hipError_t d2h(void* h, const void* d, size_t n, hipStream_t s) {
@@ -217,6 +242,7 @@ class Stager {
   char* pinned = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool busy[2] = {false, false};
+  int next = 0;  // staging buffer of the next upload chunk
   std::vector<std::thread> workers;
   std::mutex m;
   std::condition_variable cv, done;

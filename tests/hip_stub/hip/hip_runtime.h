@@ -9,7 +9,7 @@
 typedef int hipError_t;
 typedef void* hipEvent_t;
 typedef void* hipStream_t;
-enum { hipSuccess = 0 };
+enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
 enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2 };
 enum { hipEventDisableTiming = 2 };
 

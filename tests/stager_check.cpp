@@ -31,6 +31,34 @@ int main() {
         return 1;
       }
     }
+    // the split download used by the slab pipeline: pieces through
+    // alternating slots, each finished after the next one is issued
+    {
+      const size_t piece = ((size_t)5 << 20) + 123;
+      const size_t npieces = N / piece;
+      memset(back.data(), 0, N);
+      for (size_t k = 0; k <= npieces; ++k) {
+        if (k < npieces && st.d2h_issue((int)(k & 1), h.data() + k * piece, piece, nullptr) != hipSuccess) return 1;
+        if (k > 0 && st.d2h_finish(back.data() + (k - 1) * piece, (int)((k - 1) & 1), piece) != hipSuccess) return 1;
+      }
+      if (memcmp(back.data(), h.data(), npieces * piece) != 0) {
+        printf("split download mismatch\n");
+        return 1;
+      }
+      if (st.d2h_issue(2, h.data(), 16, nullptr) == hipSuccess ||
+          st.d2h_issue(0, h.data(), wd::Stager::CHUNK + 1, nullptr) == hipSuccess) {
+        printf("split download accepted a bad slot or size\n");
+        return 1;
+      }
+      // uploads of pieces smaller than a chunk, back to back (buffers alternate across calls)
+      memset(d.data(), 0, N);
+      for (size_t k = 0; k < npieces; ++k)
+        if (st.h2d(d.data() + k * piece, h.data() + k * piece, piece, nullptr) != hipSuccess) return 1;
+      if (memcmp(d.data(), h.data(), npieces * piece) != 0) {
+        printf("piecewise upload mismatch\n");
+        return 1;
+      }
+    }
     // checksum: reproducible, and any single changed byte changes it (first,
     // part boundaries, last full word, the tail), for the parallel path (n >=
     // 64 MiB) and the single-thread one

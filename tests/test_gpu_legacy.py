@@ -346,3 +346,40 @@ def test_gauge_reuse_modes(oracle, libs, prec):
         finally:
             lg.set_gauge_reuse(0)
             lg.free_sse_su3dslash()
+
+
+@pytest.mark.parametrize("prec,nd", [(32, 4), (64, 4), (32, 3)])
+def test_legacy_slab_pipeline_large(oracle, libs, prec, nd):
+    """At 32^3 x 64 the host-pointer call moves the source checkerboard up and
+    the result back in slabs of whole t (4D) / z (3D) slices, computing each
+    target slab once its neighbouring source slabs are in: every isign and
+    source checkerboard, the gauge uploaded per call and reused, and psi
+    aliasing res, against the oracle."""
+    lg = libs[prec]
+    dt = lg.dtype
+    latt = (32, 32, 32, 64)
+    print(f"[legacy slabs] prec={prec} nd={nd}", flush=True)
+    u = gen_gauge_host(latt, nd, seed=6, dtype=dt)
+    psi = gen_spinor_host(latt, nd, seed=6, dtype=dt)
+    init = lg.init_sse_su3dslash if nd == 4 else lg.init_sse_su3dslash_3d
+    free = lg.free_sse_su3dslash if nd == 4 else lg.free_sse_su3dslash_3d
+    apply = lg.sse_su3dslash_wilson if nd == 4 else lg.sse_su3dslash_wilson_3d
+    reuse = lg.set_gauge_reuse if nd == 4 else lg.set_gauge_reuse_3d
+    tol = 1e-5 if prec == 32 else 1e-13
+    vh = volume(latt) // 2
+    init(latt)
+    try:
+        for mode in (0, 2):
+            assert reuse(mode) == 0
+            for isign, cb in ((1, 0), (-1, 1), (-1, 0), (1, 1)):
+                res = np.full_like(psi, 3.0)
+                apply(u, psi, res, isign, cb)
+                want = oracle.dslash(u, psi, np.full_like(psi, 3.0), latt, isign, cb, nd=nd)
+                assert R.rel_err(res, want) < tol, (mode, isign, cb)
+                assert np.all(res.reshape(-1, 24)[cb * vh:(cb + 1) * vh] == 3.0)
+        inplace = psi.copy()
+        apply(u, inplace, inplace, 1, 0)
+        assert R.rel_err(inplace, oracle.dslash(u, psi, psi.copy(), latt, 1, 0, nd=nd)) < tol
+    finally:
+        reuse(0)
+        free()

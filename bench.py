@@ -489,6 +489,35 @@ def cg_leg(latt=(48, 48, 48, 96), kappa=0.12, tol=1e-10):
     return out
 
 
+def half_pmc_workload(reps: int = 20):
+    """For rocprofv3 --pmc runs: the 16-bit operators at 32^3 x 64 (F16 with 12- and
+    18-real links, BF16 with 12-real links; one site per thread, 64-thread blocks,
+    XCD order) and the FP32 one-site kernel for comparison, reps launches each with
+    fixed shapes (no tuning)."""
+    import ctypes as C
+    import torch
+    from sse_wilson_dslash_amd import BF16, F16, F32
+    from sse_wilson_dslash_amd._lib import check
+    from sse_wilson_dslash_amd.device import WilsonDslash
+
+    for prec, G in ((F16, 12), (F16, 18), (BF16, 12), (F32, 18)):
+        op = WilsonDslash(LATT, 4, prec, G)
+        b, r, m = C.c_int(), C.c_int(), C.c_int()
+        check(op.lib.dslash_get_launch(4, prec, G, C.byref(b), C.byref(r)))
+        check(op.lib.dslash_get_launch_march(4, prec, G, C.byref(m)))
+        check(op.lib.dslash_set_launch(4, prec, G, 64, 1))
+        check(op.lib.dslash_set_launch_march(4, prec, G, 1))
+        op.gen_gauge(seed=1)
+        psi = op.gen_spinor(seed=1)
+        res = op.new_spinor(zero=True)
+        ms = op.time_apply(res, psi, reps)
+        log(f"[pmc-half] prec={prec} G={G}: {ms * 1e3:.1f} us/launch, {op.bytes_per_call()} B/launch")
+        check(op.lib.dslash_set_launch(4, prec, G, b.value, r.value))
+        check(op.lib.dslash_set_launch_march(4, prec, G, m.value))
+        del op, psi, res
+        torch.cuda.empty_cache()
+
+
 def mrhs_pmc_workload(warmup: int = 8, reps: int = 30):
     """For rocprofv3 --pmc runs: the 32-RHS 24^3x48 launch with the two LDS-staged
     shapes fixed (128-site rows, then 12x4x4 boxes at 192 threads), each warmup +
@@ -647,6 +676,8 @@ def main() -> int:
     ap.add_argument("--no-mrhs", action="store_true", help="skip the batched Dslash legs")
     ap.add_argument("--pmc-mrhs", action="store_true",
                     help="only the 32-RHS launch in two fixed LDS-staged shapes (for rocprofv3 --pmc runs)")
+    ap.add_argument("--pmc-half", action="store_true",
+                    help="only the 16-bit and FP32 one-site kernels in fixed shapes (for rocprofv3 --pmc runs)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="only a 2 GiB calibration copy and the headline kernel (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -667,6 +698,9 @@ def main() -> int:
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
     if args.pmc_mrhs:
         mrhs_pmc_workload()
+        return 0
+    if args.pmc_half:
+        half_pmc_workload()
         return 0
     if args.pmc_workload:
         copy_leg(2.0, 3)

@@ -459,3 +459,103 @@ def test_precision_conversion_round_trip(p16, tol):
         scale = np.maximum(np.max(np.abs(ref), axis=1, keepdims=True), 1e-300)
         assert np.max(np.abs(got - ref) / scale) < t
         assert np.all(got[7] == 0.0)
+
+
+@pytest.mark.parametrize("nd,latt,prec", [(4, (8, 4, 6, 10), F64), (4, (8, 4, 6, 10), F32), (3, (4, 6, 8, 5), F32),
+                                          (4, (8, 8, 8, 8), F16)])
+def test_site_ranges(nd, latt, prec):
+    """dslash_apply_sites / dslash_pack_spinor_sites / dslash_unpack_spinor_sites
+    (the pieces of the drop-in slab pipeline): pieces over consecutive site
+    ranges, including empty and single-site ones, give the same bytes as the
+    whole-checkerboard calls, and sites outside a range are not written."""
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    lib, L = _lib.core(), _lib.latt_arg(latt)
+    u = gen_gauge_host(latt, nd, seed=8)
+    psi = gen_spinor_host(latt, nd, seed=8)
+    op = WilsonDslash(latt, nd, prec, 18)
+    op.load_gauge(u)
+    vh = op.vh
+    st = torch.cuda.current_stream().cuda_stream
+    d_aos = torch.from_numpy(psi.reshape(-1)).cuda()
+    cuts = sorted({0, 1, 1, vh // 3, vh // 3, vh // 2 + 1, vh - 1, vh})
+    for cb in (0, 1):
+        tcb = 1 - cb
+        whole = op.new_spinor(full=False)
+        _lib.check(lib.dslash_pack_spinor(_lib.ptr(whole), cb, _lib.ptr(d_aos), F64, L, nd, prec, None, st))
+        parts = torch.full_like(whole, 0x3C)
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            _lib.check(lib.dslash_pack_spinor_sites(_lib.ptr(parts), cb, _lib.ptr(d_aos), F64, L, nd, prec, None,
+                                                    a, b, st))
+        torch.cuda.synchronize()
+        assert torch.equal(parts, whole)
+        g = _lib.ptr(op.gauge) + tcb * op.gb
+        for isign in (1, -1):
+            ref = op.new_spinor(full=False, zero=True)
+            _lib.check(lib.dslash_apply_sites(_lib.ptr(ref), _lib.ptr(whole), g, L, nd, prec, 18, isign, tcb, 0, vh,
+                                              st))
+            got = torch.full_like(ref, 0x5A)
+            sentinel = got.clone()
+            a, b = cuts[1], cuts[3]
+            _lib.check(lib.dslash_apply_sites(_lib.ptr(got), _lib.ptr(whole), g, L, nd, prec, 18, isign, tcb, a, b,
+                                              st))
+            torch.cuda.synchronize()
+            # the planes of sites [a, b) hold the result, the rest is untouched
+            nvec = {F64: 12, F32: 6, F16: 3}[prec]
+            pl = got[:nvec * 16 * vh].view(nvec, vh, 16)
+            pr = ref[:nvec * 16 * vh].view(nvec, vh, 16)
+            ps = sentinel[:nvec * 16 * vh].view(nvec, vh, 16)
+            assert torch.equal(pl[:, a:b], pr[:, a:b])
+            assert torch.equal(pl[:, :a], ps[:, :a]) and torch.equal(pl[:, b:], ps[:, b:])
+            for a2, b2 in zip(cuts[:-1], cuts[1:]):
+                _lib.check(lib.dslash_apply_sites(_lib.ptr(got), _lib.ptr(whole), g, L, nd, prec, 18, isign, tcb, a2,
+                                                  b2, st))
+            torch.cuda.synchronize()
+            assert torch.equal(got, ref)
+            # the full-lattice call (any launch shape) agrees to rounding
+            full = op.new_spinor(full=False, zero=True)
+            op.apply_cb(full, whole, isign, tcb)
+            torch.cuda.synchronize()
+            x = op.unpack_cb(full, tcb)
+            y = op.unpack_cb(ref, tcb)
+            assert R.rel_err(y, x) < (1e-13 if prec == F64 else 1e-5 if prec == F32 else 1e-3)
+            out_whole = torch.zeros(op.V * 24, dtype=torch.float64, device="cuda")
+            out_parts = torch.zeros_like(out_whole)
+            _lib.check(lib.dslash_unpack_spinor(_lib.ptr(out_whole), tcb, _lib.ptr(ref), prec, L, nd, F64, None, 0,
+                                                st))
+            for a2, b2 in zip(cuts[:-1], cuts[1:]):
+                _lib.check(lib.dslash_unpack_spinor_sites(_lib.ptr(out_parts), tcb, _lib.ptr(ref), prec, L, nd, F64,
+                                                          None, 0, a2, b2, st))
+            torch.cuda.synchronize()
+            assert torch.equal(out_parts, out_whole)
+    # ranges outside the checkerboard are rejected before any launch
+    t = op.new_spinor(full=False)
+    assert lib.dslash_apply_sites(_lib.ptr(t), _lib.ptr(t) + 16, _lib.ptr(op.gauge), L, nd, prec, 18, 1, 0, 0,
+                                  vh + 1, st) == _lib.ERR_INVALID
+    assert lib.dslash_pack_spinor_sites(_lib.ptr(t), 0, _lib.ptr(d_aos), F64, L, nd, prec, None, 5, 4,
+                                        st) == _lib.ERR_INVALID
+
+
+def test_ceiling_kernel_runs():
+    """dslash_time_ceiling moves the headline kernel's 768 B/site: it runs, its
+    output planes are the xor of the 6 + 36 input planes at each site, and bad
+    shapes are rejected."""
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    lib = _lib.core()
+    latt = (8, 8, 8, 16)
+    vh = volume(latt) // 2
+    g = torch.Generator(device="cpu").manual_seed(3)
+    a = torch.randint(0, 2**31 - 1, (6, vh, 4), generator=g, dtype=torch.int64).to(torch.int32).cuda()
+    links = torch.randint(0, 2**31 - 1, (36, vh, 4), generator=g, dtype=torch.int64).to(torch.int32).cuda()
+    out = torch.zeros_like(a)
+    ms = C.c_float()
+    _lib.check(lib.dslash_time_ceiling(_lib.ptr(out), _lib.ptr(a), _lib.ptr(links), _lib.latt_arg(latt), 64, 1, 3,
+                                       torch.cuda.current_stream().cuda_stream, C.byref(ms)))
+    assert ms.value > 0
+    want = a.clone()
+    for k in range(36):
+        want[k % 6] ^= links[k]
+    assert torch.equal(out, want)
+    assert lib.dslash_time_ceiling(_lib.ptr(out), _lib.ptr(a), _lib.ptr(links), _lib.latt_arg(latt), 96, 1, 3, None,
+                                   C.byref(ms)) == _lib.ERR_INVALID

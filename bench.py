@@ -609,10 +609,12 @@ def mrhs_leg(reps: int = 50):
     return out
 
 
-def legacy_leg(calls: int = 6):
+def legacy_leg(calls: int = 12):
     """The drop-in host-pointer API (sse_su3dslash_wilson, FP32, 32^3x64): every call
     uploads the 604 MB gauge and the source half of psi and downloads the target
-    half of res over PCIe; and the resident-gauge extension, where only spinors move."""
+    half of res over PCIe; with the opt-in gauge reuse (checksum, pointer); and the
+    resident-gauge extension, where only spinors move.  Median of `calls` calls
+    after one untimed call, alternating isign and cb; PCIe-inclusive."""
     import numpy as np
     from sse_wilson_dslash_amd.device import gen_gauge_host, gen_spinor_host
     from sse_wilson_dslash_amd.legacy import SSEDslash
@@ -624,34 +626,28 @@ def legacy_leg(calls: int = 6):
     lg.init_sse_su3dslash(LATT)
     out = {}
     fl = 1320 * (int(np.prod(LATT)) // 2)
-    try:
-        lg.sse_su3dslash_wilson(u, psi, res, 1, 0)
-        t0 = time.perf_counter()
+
+    def timed(call, key):
+        call(1, 0)
+        ts = []
         for k in range(calls):
-            lg.sse_su3dslash_wilson(u, psi, res, 1 if k % 4 < 2 else -1, k % 2)
-        ms = (time.perf_counter() - t0) / calls * 1e3
-        out["host_pointers_ms_per_call"] = round(ms, 2)
-        out["host_pointers_gflops"] = round(fl / (ms * 1e-3) / 1e9, 1)
+            t0 = time.perf_counter()
+            call(1 if k % 4 < 2 else -1, k % 2)
+            ts.append(time.perf_counter() - t0)
+        ms = float(np.median(ts)) * 1e3
+        out[key + "_ms_per_call"] = round(ms, 2)
+        out[key + "_gflops"] = round(fl / (ms * 1e-3) / 1e9, 1)
+
+    try:
+        timed(lambda isign, cb: lg.sse_su3dslash_wilson(u, psi, res, isign, cb), "host_pointers")
         # opt-in gauge reuse (sse_su3dslash_set_gauge_reuse): 1 = pointer + checksum,
-        # 2 = pointer only; the first call of each uploads, the timed ones reuse
+        # 2 = pointer only; the untimed first call uploads, the timed ones reuse
         for mode, key in ((1, "gauge_reuse_checksum"), (2, "gauge_reuse_pointer")):
             assert lg.set_gauge_reuse(mode) == 0
-            lg.sse_su3dslash_wilson(u, psi, res, 1, 0)
-            t0 = time.perf_counter()
-            for k in range(calls):
-                lg.sse_su3dslash_wilson(u, psi, res, 1 if k % 4 < 2 else -1, k % 2)
-            m = (time.perf_counter() - t0) / calls * 1e3
-            out[key + "_ms_per_call"] = round(m, 2)
-            out[key + "_gflops"] = round(fl / (m * 1e-3) / 1e9, 1)
+            timed(lambda isign, cb: lg.sse_su3dslash_wilson(u, psi, res, isign, cb), key)
         assert lg.set_gauge_reuse(0) == 0
         assert lg.gauge_upload(u, 18) == 0
-        lg.wilson_resident(psi, res, 1, 0)
-        t0 = time.perf_counter()
-        for k in range(calls):
-            lg.wilson_resident(psi, res, 1 if k % 4 < 2 else -1, k % 2)
-        ms2 = (time.perf_counter() - t0) / calls * 1e3
-        out["resident_gauge_ms_per_call"] = round(ms2, 2)
-        out["resident_gauge_gflops"] = round(fl / (ms2 * 1e-3) / 1e9, 1)
+        timed(lambda isign, cb: lg.wilson_resident(psi, res, isign, cb), "resident_gauge")
     finally:
         lg.free_sse_su3dslash()
     log(f"[legacy] host-pointer API {out['host_pointers_ms_per_call']} ms/call "

@@ -208,7 +208,6 @@ struct Legacy {
       }
       h_stage.resize(Vh * 24);
     }
-    stager.start();
     nslab = 0;
     if (canonical) {
       // slabs of whole slices, each at most one staging chunk: the first target
@@ -222,6 +221,8 @@ struct Legacy {
       S = std::min<size_t>(S, std::min<size_t>(L, MAX_SLABS));
       if (S >= 3 && (L + S - 1) / S * slice_bytes <= wd::Stager::CHUNK) nslab = (int)S;
     }
+    // with the slab pipeline two pools copy at the same time (uploads, downloads)
+    stager.start(nslab ? 2 : 1);
     if (nslab) {
       for (int k = 0; k < nslab; ++k)
         if ((e = hipEventCreateWithFlags(&ev_up[k], hipEventDisableTiming)) != hipSuccess ||
@@ -229,7 +230,7 @@ struct Legacy {
           fprintf(stderr, "%s: event creation failed: %s\n", name, hipGetErrorString(e));
           exit(1);
         }
-      stager_dn.start();
+      stager_dn.start(2);
     }
     resident_G = 0;
     dev_pending = false;

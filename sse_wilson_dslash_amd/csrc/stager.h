@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <emmintrin.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -34,7 +35,10 @@ class Stager {
   // HIP call here, the runtime may already be gone at static destruction.
   ~Stager() { stop_workers(); }
 
-  void start() {
+  // share = how many pools run copies at the same time: each pool takes that
+  // fraction of the host's threads (leaving one per pool for other work), at
+  // most 7 workers besides the calling thread.
+  void start(unsigned share = 1) {
     if (pinned) return;
     if (hipHostMalloc((void**)&pinned, 2 * CHUNK, 0) != hipSuccess) {
       pinned = nullptr;
@@ -42,8 +46,8 @@ class Stager {
     }
     for (hipEvent_t& e : ev)
       if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int n = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency()) / std::max(1u, share);
+    const int n = (int)std::min(7u, hw > 2 ? hw - 2 : 0u);
     stop = false;
     // workers wait for the NEXT job: a generation count left from an earlier
     // start/finish cycle must not look like a pending job
@@ -189,9 +193,40 @@ class Stager {
   void part(int k) {
     const size_t a = job_n * k / job_parts, b = job_n * (k + 1) / job_parts;
     if (job_dst)
-      memcpy(job_dst + a, job_src + a, b - a);
+      stream_copy(job_dst + a, job_src + a, b - a);
     else
       part_sum[k] = digest(job_src + a, b - a);
+  }
+
+  // memcpy with non-temporal 16-byte stores: the staging copies are a few MB
+  // per thread, below the size where memcpy itself stops reading the
+  // destination lines before writing them, and neither side is read again by
+  // this thread, so streaming stores save that read of host memory (the
+  // bound of the host-pointer path).  The fence makes the stores visible before
+  // the copy is reported done.
+  static void stream_copy(char* dst, const char* src, size_t n) {
+    if (n < ((size_t)64 << 10)) {
+      memcpy(dst, src, n);
+      return;
+    }
+    const size_t head = (16 - ((uintptr_t)dst & 15)) & 15;
+    memcpy(dst, src, head);
+    dst += head;
+    src += head;
+    n -= head;
+    const size_t body = n & ~(size_t)63;
+    for (size_t i = 0; i < body; i += 64) {
+      const __m128i v0 = _mm_loadu_si128((const __m128i*)(src + i));
+      const __m128i v1 = _mm_loadu_si128((const __m128i*)(src + i + 16));
+      const __m128i v2 = _mm_loadu_si128((const __m128i*)(src + i + 32));
+      const __m128i v3 = _mm_loadu_si128((const __m128i*)(src + i + 48));
+      _mm_stream_si128((__m128i*)(dst + i), v0);
+      _mm_stream_si128((__m128i*)(dst + i + 16), v1);
+      _mm_stream_si128((__m128i*)(dst + i + 32), v2);
+      _mm_stream_si128((__m128i*)(dst + i + 48), v3);
+    }
+    memcpy(dst + body, src + body, n - body);
+    _mm_sfence();
   }
 
   // 64-bit digest of a byte range: four independent lanes, each folding one

@@ -518,6 +518,49 @@ def half_pmc_workload(reps: int = 20):
         torch.cuda.empty_cache()
 
 
+def cg_lattice_pmc_workload(reps: int = 10):
+    """For rocprofv3 --pmc runs: the CG's operators at 48^3 x 96 in their default
+    launch shapes, reps launches each: FP64 18-real D psi (box tiles), the FP64
+    x + a D psi form the CG uses, and F16 12-real D psi (one site per thread); then
+    the streaming ceiling kernel at the same lattice."""
+    import ctypes as C
+    import torch
+    from sse_wilson_dslash_amd import F16, F64
+    from sse_wilson_dslash_amd._lib import check, core, latt_arg, ptr
+    from sse_wilson_dslash_amd.device import WilsonDslash, current_stream
+
+    latt = (48, 48, 48, 96)
+    for prec, G in ((F64, 18), (F16, 12)):
+        op = WilsonDslash(latt, 4, prec, G)
+        op.gen_gauge(seed=1)
+        psi = op.gen_spinor(seed=1)
+        res = op.new_spinor(zero=True)
+        op.time_apply(res, psi, 3)  # first touch of the fresh fields
+        ms = op.time_apply(res, psi, reps)
+        log(f"[pmc-48] prec={prec} G={G}: {ms * 1e3:.1f} us/launch, {op.bytes_per_call()} B/launch")
+        if prec == F64:
+            x = op.cb_view(psi, 0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                check(op.lib.dslash_apply_xpay(ptr(op.cb_view(res, 0)), ptr(op.cb_view(psi, 1)),
+                                               ptr(op.gauge), ptr(x), -0.0144, op._L, F64, 18, 1, 0,
+                                               current_stream()))
+            e1.record()
+            e1.synchronize()
+            log(f"[pmc-48] FP64 x + a D psi: {e0.elapsed_time(e1) / reps * 1e3:.1f} us/launch")
+        del op, psi, res
+        torch.cuda.empty_cache()
+    vh = 48 * 48 * 48 * 96 // 2
+    a = torch.ones(96 * vh, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    links = torch.ones(576 * vh, dtype=torch.uint8, device="cuda")
+    msf = C.c_float()
+    check(core().dslash_time_ceiling(ptr(b), ptr(a), ptr(links), latt_arg(latt), 64, 1, reps, current_stream(),
+                                     C.byref(msf)))
+    log(f"[pmc-48] ceiling: {msf.value * 1e3:.1f} us/launch, {768 * vh} B/launch")
+
+
 def mrhs_pmc_workload(warmup: int = 8, reps: int = 30):
     """For rocprofv3 --pmc runs: the 32-RHS 24^3x48 launch with the two LDS-staged
     shapes fixed (128-site rows, then 12x4x4 boxes at 192 threads), each warmup +
@@ -672,6 +715,8 @@ def main() -> int:
     ap.add_argument("--no-mrhs", action="store_true", help="skip the batched Dslash legs")
     ap.add_argument("--pmc-mrhs", action="store_true",
                     help="only the 32-RHS launch in two fixed LDS-staged shapes (for rocprofv3 --pmc runs)")
+    ap.add_argument("--pmc-cg-lattice", action="store_true",
+                    help="only the CG's operators and the ceiling kernel at 48^3x96 (for rocprofv3 --pmc runs)")
     ap.add_argument("--pmc-half", action="store_true",
                     help="only the 16-bit and FP32 one-site kernels in fixed shapes (for rocprofv3 --pmc runs)")
     ap.add_argument("--pmc-workload", action="store_true",
@@ -697,6 +742,9 @@ def main() -> int:
         return 0
     if args.pmc_half:
         half_pmc_workload()
+        return 0
+    if args.pmc_cg_lattice:
+        cg_lattice_pmc_workload()
         return 0
     if args.pmc_workload:
         copy_leg(2.0, 3)

@@ -191,7 +191,7 @@ k_dslash(ApplyArgs a) {
           n2 += acc.s[sp][k].re * acc.s[sp][k].re + acc.s[sp][k].im * acc.s[sp][k].im;
         }
       s2 = (double)n2;
-      store_spinor<P, false>(a.res, a.res_norm, idx, a.g.vh, acc);
+      store_spinor<P, true>(a.res, a.res_norm, idx, a.g.vh, acc);
     } else {
       store_spinor<P, true>(a.res, a.res_norm, idx, a.g.vh, acc);
     }

@@ -496,7 +496,7 @@ __device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uin
             n2 += acc[0].s[sp][cc].re * acc[0].s[sp][cc].re + acc[0].s[sp][cc].im * acc[0].s[sp][cc].im;
           }
         if constexpr (EPI == EPI_XPAY_NORM) s2 += (double)n2;
-        store_spinor<P, false>(L.res, nullptr, idx, vh, acc[0]);
+        store_spinor<P, true>(L.res, nullptr, idx, vh, acc[0]);
       }
     }
     __syncthreads();  // buffer cur ^ 1 complete; buffer cur free for the next step's writes

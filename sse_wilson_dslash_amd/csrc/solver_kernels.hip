@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(TPB) k_zero(void* y, uint32_t vh) {
   for (int s = 0; s < 4; ++s)
 #pragma unroll
     for (int c = 0; c < 3; ++c) z.s[s][c] = {0, 0};
-  store_spinor<P, false>((v4u*)y, vnorm<P>(y, vh), i, vh, z);
+  store_spinor<P, true>((v4u*)y, vnorm<P>(y, vh), i, vh, z);
 }
 
 template <int PY, int PX>
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(TPB) k_copy(void* y, const void* x, uint32_t v
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       o.s[s][c] = {(typename Fmt<PY>::compute)a.s[s][c].re, (typename Fmt<PY>::compute)a.s[s][c].im};
-  store_spinor<PY, false>((v4u*)y, vnorm<PY>(y, vh), i, vh, o);
+  store_spinor<PY, true>((v4u*)y, vnorm<PY>(y, vh), i, vh, o);
 }
 
 template <int P>
@@ -150,8 +150,8 @@ __global__ void __launch_bounds__(TPB) k_cg_update(void* x, const void* p, void*
         rv.s[s][c].im -= a * av.s[s][c].im;
         s2 += rv.s[s][c].re * rv.s[s][c].re + rv.s[s][c].im * rv.s[s][c].im;
       }
-    store_spinor<PX, false>((v4u*)x, vnorm<PX>(x, vh), i, vh, xv);
-    store_spinor<PS, false>((v4u*)r, vnorm<PS>(r, vh), i, vh, rv);
+    store_spinor<PX, true>((v4u*)x, vnorm<PX>(x, vh), i, vh, xv);
+    store_spinor<PS, true>((v4u*)r, vnorm<PS>(r, vh), i, vh, rv);
     acc += (double)s2;
   }
   acc = block_sum(acc, lds);
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(TPB) k_xpay_beta(const void* r, void* p, uint3
       pv.s[s][c].re = rv.s[s][c].re + beta * pv.s[s][c].re;
       pv.s[s][c].im = rv.s[s][c].im + beta * pv.s[s][c].im;
     }
-  store_spinor<P, false>((v4u*)p, vnorm<P>(p, vh), i, vh, pv);
+  store_spinor<P, true>((v4u*)p, vnorm<P>(p, vh), i, vh, pv);
 }
 
 // p <- p - rp * r  (complex rp)
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(TPB) k_caxpy_rp(const void* r, void* p, uint32
       pv.s[s][c].re -= ar * rv.s[s][c].re - ai * rv.s[s][c].im;
       pv.s[s][c].im -= ar * rv.s[s][c].im + ai * rv.s[s][c].re;
     }
-  store_spinor<P, false>((v4u*)p, vnorm<P>(p, vh), i, vh, pv);
+  store_spinor<P, true>((v4u*)p, vnorm<P>(p, vh), i, vh, pv);
 }
 
 // y += x ; x <- 0
@@ -221,8 +221,8 @@ __global__ void __launch_bounds__(TPB) k_accumulate(void* y, void* x, uint32_t v
       yv.s[s][c].im += (typename Fmt<PY>::compute)xv.s[s][c].im;
       z.s[s][c] = {0, 0};
     }
-  store_spinor<PY, false>((v4u*)y, vnorm<PY>(y, vh), i, vh, yv);
-  store_spinor<PX, false>((v4u*)x, vnorm<PX>(x, vh), i, vh, z);
+  store_spinor<PY, true>((v4u*)y, vnorm<PY>(y, vh), i, vh, yv);
+  store_spinor<PX, true>((v4u*)x, vnorm<PX>(x, vh), i, vh, z);
 }
 
 // out <- b - Ay ; partial |out|^2
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(TPB) k_residual(void* out, const void* b, cons
         bv.s[s][c].im -= av.s[s][c].im;
         s2 += bv.s[s][c].re * bv.s[s][c].re + bv.s[s][c].im * bv.s[s][c].im;
       }
-    store_spinor<P, false>((v4u*)out, vnorm<P>(out, vh), i, vh, bv);
+    store_spinor<P, true>((v4u*)out, vnorm<P>(out, vh), i, vh, bv);
     acc += (double)s2;
   }
   acc = block_sum(acc, lds);
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(TPB) k_axpby(double a, const void* x, double b
       yv.s[s][c].re = (T)a * xv.s[s][c].re + (T)b * yv.s[s][c].re;
       yv.s[s][c].im = (T)a * xv.s[s][c].im + (T)b * yv.s[s][c].im;
     }
-  store_spinor<P, false>((v4u*)y, vnorm<P>(y, vh), i, vh, yv);
+  store_spinor<P, true>((v4u*)y, vnorm<P>(y, vh), i, vh, yv);
 }
 
 __global__ void __launch_bounds__(1024) k_finalize(const double* partial, int nparts, double* scal, int op) {

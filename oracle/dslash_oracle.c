@@ -1,9 +1,9 @@
 /*
  * CPU restatement of the reference Wilson Dslash -- TEST INFRASTRUCTURE ONLY.
  *
- * This file is the oracle: the checker that tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg compare the HIP library against.  It is never
- * linked into, loaded by or called from the product libraries.
+ * This file is the oracle: the checker that tests/ and __graft_entry__.smoke()
+ * compare the HIP library and the CPU build against.  It is never linked into,
+ * loaded by or called from the product libraries.
  *
  * It restates, in plain C with OpenMP over target sites, what the reference's
  * scalar path computes (JeffersonLab/sse_wilson_dslash):

@@ -1,7 +1,7 @@
 """ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
 
-The oracle is the checker: only tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py import it.  The product libraries never load it.
+The oracle is the checker: only tests/ and __graft_entry__.smoke() import it.
+The product libraries never load it.
 """
 from __future__ import annotations
 

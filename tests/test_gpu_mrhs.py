@@ -241,12 +241,24 @@ def test_autotuner(oracle):
     op, u = items[2][0].op, items[2][1]
     psi = op.gen_spinor(seed=7)
     res = op.new_spinor(zero=True)
-    t = tune_apply(op, res, psi)
-    assert t["block_threads"] in (64, 128, 192, 256) and t["xcd_remap"] in (0, 1) and t["ms"] > 0
-    assert t["march"] in (1, 4, 8)
-    # the tuned shape still computes the same operator
-    op.apply(res, psi, 1, 0)
-    got = op.unpack_spinor(res, dtype=np.float32)
-    h = op.unpack_spinor(psi, dtype=np.float32)
-    want = oracle.dslash(u, h, np.zeros_like(h), op.latt, 1, 0)
-    assert R.rel_err(got[op.vh:], want[op.vh:]) < 1e-5
+    # the tuner rewrites the process-wide single-RHS entry of (4D, F32, 18 reals);
+    # later tests expect the default shape, so it is restored afterwards
+    import ctypes as C
+    from sse_wilson_dslash_amd import _lib
+    lib = _lib.core()
+    b0, r0, m0 = C.c_int(), C.c_int(), C.c_int()
+    _lib.check(lib.dslash_get_launch(4, F32, 18, C.byref(b0), C.byref(r0)))
+    _lib.check(lib.dslash_get_launch_march(4, F32, 18, C.byref(m0)))
+    try:
+        t = tune_apply(op, res, psi)
+        assert t["block_threads"] in (64, 128, 192, 256) and t["xcd_remap"] in (0, 1) and t["ms"] > 0
+        assert t["march"] in (1, 4, 8)
+        # the tuned shape still computes the same operator
+        op.apply(res, psi, 1, 0)
+        got = op.unpack_spinor(res, dtype=np.float32)
+        h = op.unpack_spinor(psi, dtype=np.float32)
+        want = oracle.dslash(u, h, np.zeros_like(h), op.latt, 1, 0)
+        assert R.rel_err(got[op.vh:], want[op.vh:]) < 1e-5
+    finally:
+        _lib.check(lib.dslash_set_launch(4, F32, 18, b0.value, r0.value))
+        _lib.check(lib.dslash_set_launch_march(4, F32, 18, m0.value))

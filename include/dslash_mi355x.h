@@ -200,7 +200,10 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* problems, int nprob, int prec, 
  *      (y, z) (chosen per lattice to fit block_threads), so x hops and the y / z
  *      hops inside the box read the LDS slab (F32, rhs_block 1, 2, block_threads
  *      >= 128 and >= Lx/2 of every lattice, same launch conditions as 8; others
- *      fall back to 8, then 4).
+ *      fall back to 8, then 4).  With rhs_block 1, block_threads may go up to 768
+ *      (wide tiles: one block per CU, larger boxes, fewer hops leaving the box;
+ *      all fields of a box run back to back); a launch that cannot use box tiles
+ *      then runs its fallback with 256 threads.
  * set: same arguments; the autotuner also sets them. */
 int dslash_get_mrhs_launch(int prec, int gauge_reals, int persistent, int* rhs_block, int* block_threads,
                            int* grid_blocks, int* t_block);

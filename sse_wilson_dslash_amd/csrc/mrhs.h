@@ -16,14 +16,19 @@ constexpr int MRHS_LDS_THREADS = 128;  // threads (= spatial sites) per LDS-stag
 constexpr int MRHS_LDS_GROUP = 8;    // RHS blocks of one spatial block run together
 constexpr int MRHS_T_BOX = 16;       // t_block value of LDS-staged box tiles
 constexpr int MRHS_BOX_SLICES = 8;   // t-slices per box tile
+constexpr int MRHS_BOX_WIDE = 768;   // most threads of a wide box tile (FP32, one field per tile)
 
 // Tile shapes: t_block 1 (one site per thread, any rhs_block), MRHS_T_BLOCK
 // (t-marching, rhs_block <= 2), MRHS_T_BLOCK_LDS (t-marching with an LDS slab,
 // F32, rhs_block <= 2, MRHS_LDS_THREADS threads), MRHS_T_BOX (LDS-staged box
 // tiles, F32 with rhs_block <= 2 or F64 with rhs_block 1, >= 128 threads; the
-// single-field path also takes F64S).
+// single-field path also takes F64S).  Blocks hold 64..256 threads, except wide
+// box tiles: F32, rhs_block x threads <= MRHS_BOX_WIDE (one block per CU, 144 KiB
+// of LDS at 768 field-sites).
 inline bool mrhs_shape_ok(int prec, int R, int block, int t_block) {
-  if (!(R == 1 || R == 2 || R == 4) || block < 64 || block > 256 || block % 64) return false;
+  if (!(R == 1 || R == 2 || R == 4) || block < 64 || block % 64) return false;
+  if (block > 256)
+    return t_block == MRHS_T_BOX && prec == F32 && R <= 2 && R * block <= MRHS_BOX_WIDE;
   if (t_block == 1) return true;
   if (t_block == MRHS_T_BLOCK) return R <= 2;
   if (t_block == MRHS_T_BLOCK_LDS) return prec == F32 && R <= 2 && block == MRHS_LDS_THREADS;

@@ -202,8 +202,8 @@ __device__ __forceinline__ void lds_hop(const MrhsLattice& L, v4u (*slab)[6][MRH
 // RHS block is always full).  Every thread of the block executes the same
 // barriers: the step loop bound and the tile are uniform over the block.
 template <int G, int IS, int R>
-__device__ void mrhs_march_lds(const MrhsLattice& L, v4u (*slab)[R][6][MRHS_LDS_THREADS], uint32_t s0, int t0,
-                               int r0) {
+__device__ __forceinline__ void mrhs_march_lds(const MrhsLattice& L, v4u (*slab)[R][6][MRHS_LDS_THREADS],
+                                               uint32_t s0, int t0, int r0) {
   constexpr int BS = MRHS_LDS_THREADS;
   constexpr int TB = MRHS_T_BLOCK_LDS;
   constexpr int PV = PairVec<F32, G>::value;
@@ -393,8 +393,8 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
 // other RHS blocks of the same box.  EPI (one field only): EPI_XPAY stores
 // x + a D psi, EPI_XPAY_NORM also adds each of the lane's |res|^2 to s2.
 template <int P, int G, int IS, int R, bool NTL, int EPI>
-__device__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0, int r0,
-                              const BoxEpi& e, double& s2) {
+__device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0,
+                                              int r0, const BoxEpi& e, double& s2) {
   static_assert(EPI == EPI_PLAIN || R == 1, "the epilogue forms take one field");
   typedef typename Fmt<P>::compute T;
   constexpr int TB = MRHS_BOX_SLICES;
@@ -520,8 +520,9 @@ __device__ inline void box_block_sum(double v, double* out) {
   }
 }
 
-template <int P, int G, int R, int IS, bool NTL, int EPI = EPI_PLAIN>
-__global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list, BoxEpi e) {
+// MAXT: the launch bound (256, or MRHS_BOX_WIDE for wide FP32 one-field tiles).
+template <int P, int G, int R, int IS, bool NTL, int EPI = EPI_PLAIN, int MAXT = 256>
+__global__ void __launch_bounds__(MAXT) k_mrhs_box(MrhsList list, BoxEpi e) {
   extern __shared__ v4u box_slab[];
   const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
   const uint32_t total = list.total_tiles;
@@ -600,6 +601,23 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
     if constexpr (P == F64) {  // FP64: one field per thread
       if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
       else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
+    } else if (block > 256) {  // wide tiles: R x block <= MRHS_BOX_WIDE
+      constexpr int W1 = MRHS_BOX_WIDE, W2 = MRHS_BOX_WIDE / 2;
+      if (R == 1) {
+        if (plus)
+          hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false, EPI_PLAIN, W1>), dim3(grid), dim3(block), lds, s, list,
+                             BoxEpi{});
+        else
+          hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false, EPI_PLAIN, W1>), dim3(grid), dim3(block), lds, s, list,
+                             BoxEpi{});
+      } else {
+        if (plus)
+          hipLaunchKernelGGL((k_mrhs_box<P, G, 2, 1, false, EPI_PLAIN, W2>), dim3(grid), dim3(block), lds, s, list,
+                             BoxEpi{});
+        else
+          hipLaunchKernelGGL((k_mrhs_box<P, G, 2, -1, false, EPI_PLAIN, W2>), dim3(grid), dim3(block), lds, s, list,
+                             BoxEpi{});
+      }
     } else {
       if (R == 1) {
         if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
@@ -654,7 +672,8 @@ int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R
              "1/2/4; t_block %d takes rhs_block 1/2; t_block %d takes F32, rhs_block 1/2 and %d threads; "
              "t_block %d takes F32, rhs_block 1/2 and >= 128 threads",
              R, block, t_block, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS, MRHS_T_BOX);
-  WD_REQUIRE(block >= 64 && block <= 256 && block % 64 == 0, "block_threads must be 64..256, multiple of 64");
+  WD_REQUIRE(block >= 64 && block % 64 == 0 && (block <= 256 || t_block == MRHS_T_BOX),
+             "block_threads must be 64..256, multiple of 64 (wide box tiles: up to %d)", MRHS_BOX_WIDE);
   memset(list, 0, sizeof(*list));
   uint64_t tiles = 0;
   for (int i = 0; i < nprob; ++i) {
@@ -696,7 +715,10 @@ int mrhs_build_list(const dslash_mrhs_problem* probs, int nprob, int prec, int R
       L.nty = (p.latt[1] + L.ty - 1) / L.ty;
       L.nsb = L.nty * ((p.latt[2] + L.tz - 1) / L.tz);
       L.ntc = (p.latt[3] + MRHS_BOX_SLICES - 1) / MRHS_BOX_SLICES;
-      if (L.nrb > MRHS_LDS_GROUP && L.nrb % MRHS_LDS_GROUP == 0) L.rg = MRHS_LDS_GROUP;
+      // A wide tile fills a CU, so an XCD runs ~32 at once: all fields of a box
+      // in a row then share one fetch of its links.  256-thread tiles run two or
+      // more per CU and take the fields of a box in groups.
+      if (block <= 256 && L.nrb > MRHS_LDS_GROUP && L.nrb % MRHS_LDS_GROUP == 0) L.rg = MRHS_LDS_GROUP;
     } else {
       L.nsb = (L.S + block - 1) / block;
       L.ntc = (p.latt[3] + t_block - 1) / t_block;

@@ -90,6 +90,7 @@ int dslash_apply_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int
                                  !mrhs_lds_compatible(probs, nprob, R)))
     tb = MRHS_T_BLOCK;
   if (!mrhs_shape_ok(prec, R, block, tb)) tb = 1;
+  if (tb != MRHS_T_BOX && block > 256) block = 256;  // a wide box shape that this launch cannot run
   return mrhs_run(probs, nprob, prec, G, R, block, grid_blocks, tb, (hipStream_t)stream);
 }
 
@@ -112,11 +113,12 @@ int dslash_set_mrhs_launch(int prec, int G, int persistent, int rhs_block, int b
   WD_REQUIRE(prec == F32 || prec == F64, "multi-RHS Dslash is built for F32 and F64");
   WD_REQUIRE(G == 18 || G == 12, "gauge_reals must be 18 or 12");
   WD_REQUIRE(rhs_block == 1 || rhs_block == 2 || rhs_block == 4, "rhs_block must be 1, 2 or 4");
-  WD_REQUIRE(block_threads >= 64 && block_threads <= 256 && block_threads % 64 == 0,
-             "block_threads must be 64, 128, 192 or 256");
+  WD_REQUIRE(block_threads >= 64 && block_threads <= MRHS_BOX_WIDE && block_threads % 64 == 0,
+             "block_threads must be a multiple of 64 in 64..256 (wide box tiles: up to %d)", MRHS_BOX_WIDE);
   WD_REQUIRE(mrhs_shape_ok(prec, rhs_block, block_threads, t_block),
              "t_block must be 1, %d (rhs_block 1 or 2), %d (F32, rhs_block 1 or 2, %d threads) or %d (F32, "
-             "rhs_block 1 or 2, >= 128 threads)", MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS, MRHS_T_BOX);
+             "rhs_block 1 or 2, >= 128 threads; rhs_block 1 above 256 threads, up to %d)", MRHS_T_BLOCK,
+             MRHS_T_BLOCK_LDS, MRHS_LDS_THREADS, MRHS_T_BOX, MRHS_BOX_WIDE);
   WD_REQUIRE(persistent ? grid_blocks > 0 : grid_blocks == 0, "grid_blocks must be > 0 exactly when persistent");
   mrhs_entry(prec, G, persistent) = MrhsCfg{rhs_block, block_threads, grid_blocks, t_block};
   return 0;
@@ -196,7 +198,7 @@ int dslash_tune_mrhs(const dslash_mrhs_problem* probs, int nprob, int prec, int 
   for (int tb : {1, MRHS_T_BLOCK, MRHS_T_BLOCK_LDS, MRHS_T_BOX})
     for (int R : {1, 2, 4}) {
       if (tb == MRHS_T_BLOCK_LDS && !mrhs_lds_compatible(probs, nprob, R)) continue;
-      for (int block : {64, 128, 192, 256})
+      for (int block : {64, 128, 192, 256, 384, 512, 768})
         for (int gi = 0; gi < ng; ++gi) {
           if (!mrhs_shape_ok(prec, R, block, tb)) continue;
           if (tb == MRHS_T_BOX && !mrhs_box_compatible(probs, nprob, R, block)) continue;

@@ -143,6 +143,48 @@ def test_box_tiles(oracle, launch_table, rhs_block, nrhs, block, latt, isign):
     check_item(oracle, item, u, host, 1e-5, ks=[0, nrhs - 1])
 
 
+@pytest.mark.parametrize("rhs_block,block", [(1, 512), (1, 768), (2, 384)])
+@pytest.mark.parametrize("latt,nrhs", [((8, 8, 8, 16), 4), ((24, 24, 24, 16), 6), ((12, 6, 4, 8), 2),
+                                       ((24, 6, 10, 4), 4)])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_wide_box_tiles(oracle, launch_table, rhs_block, block, latt, nrhs, isign):
+    """t_block 16 with more than 256 threads (rhs_block x threads <= 768): on
+    24^3 x 16 the boxes are 12 x 8 x 8 (768), 12 x 6 x 6 (512) or 12 x 4 x 8 (384)
+    sites with hops leaving the box in y and z; on the small lattices one box spans
+    the slice (wrap inside the box, idle lanes); T = 4 / 8 / 16 give one or two
+    t-chunks.  Against the single-RHS kernel and the oracle."""
+    print(f"[wide box tiles] latt={latt} block={block} R={rhs_block} nrhs={nrhs} isign={isign}", flush=True)
+    item, u, host = make_item(latt, nrhs, isign=isign, target_cb=0)
+    set_launch(F32, 18, False, rhs_block, block, 0, 16)
+    apply_batch([item], rhs_block=rhs_block, block_threads=block)
+    torch.cuda.synchronize()
+    op = item.op
+    for k in range(nrhs):
+        single = op.new_spinor(False)
+        op.apply_cb(single, item.psi.field(k), isign, 0)
+        a = op.unpack_cb(single, 0, dtype=np.float32)
+        b = op.unpack_cb(item.res.field(k), 0, dtype=np.float32)
+        assert R.rel_err(b, a) < 1e-6, (k, R.rel_err(b, a))
+    check_item(oracle, item, u, host, 1e-5, ks=[0, nrhs - 1])
+
+
+def test_wide_box_fallback(oracle, launch_table):
+    """A tabled wide shape (768 threads, one field) on a launch that cannot take
+    box tiles (mixed isign) runs its fallback with 256 threads; rhs_block 2 with
+    more than 384 threads and FP64 above 256 are rejected."""
+    from sse_wilson_dslash_amd._lib import DslashError
+    set_launch(F32, 18, False, 1, 768, 0, 16)
+    mixed = uneven_items()
+    apply_batch([it[0] for it in mixed])
+    torch.cuda.synchronize()
+    for item, u, host in mixed:
+        check_item(oracle, item, u, host, 1e-5, ks=[0, 3])
+    with pytest.raises(DslashError):
+        set_launch(F32, 18, False, 2, 512, 0, 16)
+    with pytest.raises(DslashError):
+        set_launch(F64, 18, False, 1, 512, 0, 16)
+
+
 def test_box_tiles_persistent_and_fallback(oracle, launch_table):
     """A persistent box-tile launch over three lattices (4, 8 and 12 sites per x
     row) with one isign; a lattice with Lx/2 > block_threads falls back to the

@@ -562,9 +562,10 @@ def cg_lattice_pmc_workload(reps: int = 10):
 
 
 def mrhs_pmc_workload(warmup: int = 8, reps: int = 30):
-    """For rocprofv3 --pmc runs: the 32-RHS 24^3x48 launch with the two LDS-staged
-    shapes fixed (128-site rows, then 12x4x4 boxes at 192 threads), each warmup +
-    reps launches, no tuning."""
+    """For rocprofv3 --pmc runs: the 32-RHS 24^3x48 launch with the LDS-staged
+    shapes fixed (128-site rows; 12x4x4 boxes at 192 threads, two fields per tile;
+    12x8x8 wide boxes at 768 threads, one field per tile), each warmup + reps
+    launches, no tuning."""
     import torch
     from sse_wilson_dslash_amd import F32
     from sse_wilson_dslash_amd.batch import BatchItem, RhsBatch, apply_batch, get_launch, set_launch
@@ -578,7 +579,7 @@ def mrhs_pmc_workload(warmup: int = 8, reps: int = 30):
     items = [BatchItem(op, psi, RhsBatch(op, 32, zero=True), 1, 0)]
     saved = get_launch(F32, 18, False)
     try:
-        for R, block, t_block in ((2, 128, 8), (2, 192, 16)):
+        for R, block, t_block in ((2, 128, 8), (2, 192, 16), (1, 768, 16)):
             set_launch(F32, 18, False, R, block, 0, t_block)
             for _ in range(warmup):
                 apply_batch(items)

@@ -520,9 +520,9 @@ __device__ inline void box_block_sum(double v, double* out) {
   }
 }
 
-// MAXT: the launch bound (256, or MRHS_BOX_WIDE for wide FP32 one-field tiles).
-template <int P, int G, int R, int IS, bool NTL, int EPI = EPI_PLAIN, int MAXT = 256>
-__global__ void __launch_bounds__(MAXT) k_mrhs_box(MrhsList list, BoxEpi e) {
+// The box-tile grid: each block walks its XCD's contiguous share of the tiles.
+template <int P, int G, int R, int IS, bool NTL, int EPI>
+__device__ __forceinline__ void mrhs_box_grid(const MrhsList& list, const BoxEpi& e) {
   extern __shared__ v4u box_slab[];
   const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
   const uint32_t total = list.total_tiles;
@@ -546,6 +546,17 @@ __global__ void __launch_bounds__(MAXT) k_mrhs_box(MrhsList list, BoxEpi e) {
                                          (int)(grp * L.rg + rbi) * R, e, s2);
     if constexpr (EPI == EPI_XPAY_NORM) box_block_sum(s2, e.norm_partial + 2 * (size_t)t);
   }
+}
+
+template <int P, int G, int R, int IS, bool NTL, int EPI = EPI_PLAIN>
+__global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list, BoxEpi e) {
+  mrhs_box_grid<P, G, R, IS, NTL, EPI>(list, e);
+}
+
+// Wide FP32 tiles: R x blockDim.x <= MRHS_BOX_WIDE (one block per CU), cached links.
+template <int G, int R, int IS>
+__global__ void __launch_bounds__(MRHS_BOX_WIDE / R) k_mrhs_box_wide(MrhsList list) {
+  mrhs_box_grid<F32, G, R, IS, false, EPI_PLAIN>(list, BoxEpi{});
 }
 
 template <int P, int G, int R, int TB>
@@ -602,21 +613,12 @@ static void launch_mrhs_pg(const MrhsList& list, int R, int block, int TB, uint3
       if (plus) hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
       else hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false>), dim3(grid), dim3(block), lds, s, list, BoxEpi{});
     } else if (block > 256) {  // wide tiles: R x block <= MRHS_BOX_WIDE
-      constexpr int W1 = MRHS_BOX_WIDE, W2 = MRHS_BOX_WIDE / 2;
       if (R == 1) {
-        if (plus)
-          hipLaunchKernelGGL((k_mrhs_box<P, G, 1, 1, false, EPI_PLAIN, W1>), dim3(grid), dim3(block), lds, s, list,
-                             BoxEpi{});
-        else
-          hipLaunchKernelGGL((k_mrhs_box<P, G, 1, -1, false, EPI_PLAIN, W1>), dim3(grid), dim3(block), lds, s, list,
-                             BoxEpi{});
+        if (plus) hipLaunchKernelGGL((k_mrhs_box_wide<G, 1, 1>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box_wide<G, 1, -1>), dim3(grid), dim3(block), lds, s, list);
       } else {
-        if (plus)
-          hipLaunchKernelGGL((k_mrhs_box<P, G, 2, 1, false, EPI_PLAIN, W2>), dim3(grid), dim3(block), lds, s, list,
-                             BoxEpi{});
-        else
-          hipLaunchKernelGGL((k_mrhs_box<P, G, 2, -1, false, EPI_PLAIN, W2>), dim3(grid), dim3(block), lds, s, list,
-                             BoxEpi{});
+        if (plus) hipLaunchKernelGGL((k_mrhs_box_wide<G, 2, 1>), dim3(grid), dim3(block), lds, s, list);
+        else hipLaunchKernelGGL((k_mrhs_box_wide<G, 2, -1>), dim3(grid), dim3(block), lds, s, list);
       }
     } else {
       if (R == 1) {

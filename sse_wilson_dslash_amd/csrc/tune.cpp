@@ -263,9 +263,8 @@ int dslash_tune_load(const char* path) {
     } else if (int n = sscanf(line, "mrhs %d %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &a[6]);
                n >= 6) {
       const int tb = n == 7 ? a[6] : 1;  // files written before t-marching tiles carry 6 fields
-      if ((a[0] == F32 || a[0] == F64) && (a[1] == 18 || a[1] == 12) && (a[3] == 1 || a[3] == 2 || a[3] == 4) &&
-          a[4] >= 64 && a[4] <= 256 && a[4] % 64 == 0 && a[5] >= 0 &&
-          mrhs_shape_ok(a[0], a[3], a[4], tb))
+      if ((a[0] == F32 || a[0] == F64) && (a[1] == 18 || a[1] == 12) && a[5] >= 0 &&
+          mrhs_shape_ok(a[0], a[3], a[4], tb))  // the RHS block, thread count and tile shape together
         mrhs_entry(a[0], a[1], a[2]) = MrhsCfg{a[3], a[4], a[2] ? (a[5] > 0 ? a[5] : 2048) : 0, tb};
       else
         ++bad;

@@ -298,17 +298,33 @@ def test_mrhs_launch_table_and_tune_file(tmp_path):
     assert lib.dslash_set_mrhs_launch(_lib.F64, 18, 0, 2, 192, 0, 16) == _lib.ERR_INVALID  # box tiles: F32 only
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 4, 192, 0, 16) == _lib.ERR_INVALID  # rhs_block <= 2
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 64, 0, 16) == _lib.ERR_INVALID  # >= 128 threads
+    # wide box tiles: rhs_block x threads <= 768, F32 box tiles only
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 768, 0, 16) == 0
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 384, 0, 16) == 0
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 512, 0, 16) == _lib.ERR_INVALID
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 832, 0, 16) == _lib.ERR_INVALID
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 512, 0, 8) == _lib.ERR_INVALID
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 512, 0, 1) == _lib.ERR_INVALID
+    assert lib.dslash_set_mrhs_launch(_lib.F64, 18, 0, 1, 512, 0, 16) == _lib.ERR_INVALID
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 300, 0, 16) == _lib.ERR_INVALID  # multiple of 64
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 2, 192, 0, 16) == 0
     assert lib.dslash_set_mrhs_launch(_lib.F32, 12, 1, 2, 128, 1024, 8) == 0
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, 1, 192, 0, 4) == 0
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 12, 0, 1, 768, 0, 16) == 0
     path = str(tmp_path / "tune.txt")
     assert lib.dslash_tune_save(path.encode()) == 0
     assert "mrhs 0 18 0 1 192 0 4\n" in open(path).read()
+    assert "mrhs 0 12 0 1 768 0 16\n" in open(path).read()
+    saved12np = [2, 192, 0, 16]
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 12, 0, *saved12np) == 0
     assert lib.dslash_set_mrhs_launch(_lib.F32, 18, 0, *saved) == 0
     assert "mrhs 0 12 1 2 128 1024 8\n" in open(path).read()
     assert lib.dslash_tune_load(path.encode()) == 0
     lib.dslash_get_mrhs_launch(_lib.F32, 18, 0, *[C.byref(x) for x in v])
     assert [x.value for x in v] == [1, 192, 0, 4]
+    lib.dslash_get_mrhs_launch(_lib.F32, 12, 0, *[C.byref(x) for x in v])
+    assert [x.value for x in v] == [1, 768, 0, 16]  # a wide shape reloads
+    assert lib.dslash_set_mrhs_launch(_lib.F32, 12, 0, *saved12np) == 0
     old = tmp_path / "old.txt"  # a file from before t-marching tiles: 6 fields, t_block 1
     old.write_text("mrhs 0 18 0 2 128 0\n")
     assert lib.dslash_tune_load(str(old).encode()) == 0

@@ -219,3 +219,38 @@ def test_cpu_result_independent_of_threads_and_blocking(tmp_path, prec, nd, latt
         outs.append(np.load(path))
     for o in outs[1:]:
         assert o.tobytes() == outs[0].tobytes()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("nd,latt", [(4, (8, 4, 6, 8)), (3, (4, 6, 4, 6))])
+def test_c_caller_on_cpu_build(oracle, tmp_path, prec, nd, latt):
+    """tests/dropin_check.c (a C caller shaped like the reference's full-lattice
+    tests, tests/testDslashFull.cc:24-117) built against the CPU library of each
+    precision: every isign / source checkerboard and the in-place call against
+    the oracle, source checkerboard of res untouched."""
+    exe = tmp_path / "dropin_cpu"
+    subprocess.run(["gcc", "-std=c11", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "include", f"f{prec}"), "-o", str(exe),
+                    os.path.join(ROOT, "tests", "dropin_check.c"), "-L", _lib.LIB_DIR,
+                    f"-l{_lib.LEGACY_CPU[prec][3:-3]}", f"-Wl,-rpath,{_lib.LIB_DIR}"],
+                   check=True, capture_output=True, timeout=120)
+    env = dict(os.environ, OMP_NUM_THREADS="2")
+    out = subprocess.run([str(exe), str(nd), *[str(x) for x in latt], str(tmp_path)], capture_output=True,
+                         text=True, timeout=120, env=env)
+    assert out.returncode == 0, out.stdout + out.stderr
+    dt = np.float32 if prec == 32 else np.float64
+    V = volume(latt)
+    vh = V // 2
+    u = np.fromfile(tmp_path / "u.bin", dtype=dt)
+    psi = np.fromfile(tmp_path / "psi.bin", dtype=dt)
+    for isign in (1, -1):
+        for cb in (0, 1):
+            got = np.fromfile(tmp_path / f"res_{isign}_{cb}.bin", dtype=dt)
+            want = oracle.dslash(u, psi, np.full(V * 24, 7.0, dtype=dt), latt, isign, cb, nd=nd)
+            t = slice((1 - cb) * vh * 24, (2 - cb) * vh * 24)
+            assert np.all(got[cb * vh * 24:(cb + 1) * vh * 24] == 7.0)
+            assert R.rel_err(got[t], want[t]) < TOL[prec], (isign, cb)
+    got = np.fromfile(tmp_path / "inplace_1_0.bin", dtype=dt)
+    want = oracle.dslash(u, psi, psi.copy(), latt, 1, 0, nd=nd)
+    assert np.array_equal(got[:vh * 24], psi[:vh * 24])
+    assert R.rel_err(got[vh * 24:], want[vh * 24:]) < TOL[prec]

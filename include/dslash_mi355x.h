@@ -151,13 +151,18 @@ int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int
  * later; 16 = LDS-staged box tiles (4D FP32, FP64, sloppy FP64): a block of block_threads
  * (>= 128, >= Lx/2) holds whole x rows of a ty x tz box of (y, z), walks 8
  * t-slices, and stages each slice's source spinors in LDS for the x hops and
- * the y / z hops inside the box (links non-temporal).  Any lattice extent is
+ * the y / z hops inside the box (links non-temporal); 32 = the same box tiles
+ * (4D FP32) whose y / z hops that leave the box read an LDS halo: each thread
+ * loads one neighbouring site of the next slice while the current one is
+ * computed and stores it projected for its hop.  Any lattice extent is
  * allowed (the last chunk is shorter).  A launch that would have fewer than
  * 131072 threads at this length (small lattices, short t or z) uses the next
  * shorter one, down to 1; box tiles that cannot run (or would hold fewer than
- * 64 sites) fall back to 8.  The x + a D psi forms run in box tiles under 16
- * and one site per thread otherwise.  dslash_set_launch leaves it
- * unchanged; dslash_tune_apply tunes it with the block shape. */
+ * 64 sites) fall back to 8, and 32 falls back to 16 for other formats or where
+ * the halo would need more than one entry per thread.  The x + a D psi forms
+ * run in box tiles under 16 and 32 and one site per thread otherwise.
+ * dslash_set_launch leaves it unchanged; dslash_tune_apply tunes it with the
+ * block shape. */
 int dslash_set_launch_march(int nd, int prec, int gauge_reals, int march);
 int dslash_get_launch_march(int nd, int prec, int gauge_reals, int* march);
 /* The marching length res = D psi actually runs with on this lattice (the set

@@ -60,8 +60,10 @@ void clear_error() {
 // (XCD-contiguous order is worth ~25%; marching 6-16% for FP64, sloppy FP64 and
 // the 3D operator, nothing for 16-bit spinors; for 4D FP32, LDS-staged box tiles
 // (march 16) are 6-7% faster than one site per thread at 32^3 x 64 and within
-// 1% at 48^3 x 96, where marching is 3-5% slower; for 4D FP64 they are 3-4%
-// and for sloppy FP64 6-10% faster than a march of 8).
+// 1% at 48^3 x 96, where marching is 3-5% slower, and box tiles with the LDS halo
+// (march 32) another 5% (18-real links) and 9% (12-real) faster at 32^3 x 64;
+// for 4D FP64 box tiles are 3-4% and for sloppy FP64 6-10% faster than a march
+// of 8).
 // Index: [nd == 4][F32, F64, F16, BF16, F64S][G == 12].
 static LaunchCfg g_cfg[2][NUM_PREC][2] = {
     {{{256, 1, 8}, {256, 1, 8}},
@@ -69,7 +71,7 @@ static LaunchCfg g_cfg[2][NUM_PREC][2] = {
      {{64, 1, 1}, {64, 1, 8}},
      {{64, 1, 1}, {64, 1, 1}},
      {{256, 1, 8}, {256, 1, 8}}},
-    {{{256, 1, 16}, {256, 1, 16}},
+    {{{256, 1, 32}, {256, 1, 32}},
      {{256, 1, 16}, {256, 1, 16}},
      {{256, 1, 1}, {256, 1, 1}},
      {{256, 1, 1}, {256, 1, 1}},
@@ -340,15 +342,18 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
 // (down to one site per thread) is used when the launch would fall below
 // MARCH_MIN_THREADS threads.
 // Box tiles (16) need the 4D FP32, FP64 or sloppy-FP64 operator, a whole x row per block and
-// the same thread floor; otherwise they fall back to a march of 8.
+// the same thread floor; otherwise they fall back to a march of 8.  Box tiles with
+// the LDS halo (32) need FP32 and a halo of at most one entry per thread; otherwise
+// they fall back to 16.
 static int effective_march(const int latt[4], int nd, int prec, int block, int march) {
   const uint64_t L = (uint64_t)latt[nd - 1];
   const uint64_t slice = (uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2 / L;
   int m = march_ok(march) ? march : 1;
-  if (m == 16) {
+  if (m == MARCH_BOX_HALO) m = (prec == F32 && box_halo_single(latt, block) > 0) ? MARCH_BOX_HALO : MARCH_BOX;
+  if (m == MARCH_BOX || m == MARCH_BOX_HALO) {
     if (nd == 4 && (prec == F32 || prec == F64 || prec == F64S) && block >= 128 &&
         (uint64_t)box_tiles_single(latt, block) * (uint64_t)block >= MARCH_MIN_THREADS)
-      return 16;
+      return m;
     m = 8;
   }
   while (m > 1 && slice * ((L + m - 1) / m) < MARCH_MIN_THREADS) m = m == 8 ? 4 : 1;
@@ -358,7 +363,8 @@ static int effective_march(const int latt[4], int nd, int prec, int block, int m
 int apply_blocks(const int latt[4], int nd, int prec, int G) {
   const uint32_t vh = (uint32_t)((uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2);
   const LaunchCfg lc = launch_cfg(nd, prec, G);
-  if (effective_march(latt, nd, prec, lc.block, lc.march) == 16) return (int)box_tiles_single(latt, lc.block);
+  const int m = effective_march(latt, nd, prec, lc.block, lc.march);
+  if (m == MARCH_BOX || m == MARCH_BOX_HALO) return (int)box_tiles_single(latt, lc.block);
   return (int)((vh + lc.block - 1) / lc.block);
 }
 
@@ -395,9 +401,9 @@ int apply_cb_ex(void* res_cb, const void* psi_cb, const void* gauge_cb, const in
   a.site0 = 0;
   a.site_end = a.g.vh;
   a.march = effective_march(latt, nd, prec, lc.block, lc.march);
-  if (a.march == 16)  // LDS-staged box tiles, every epilogue form
+  if (a.march == MARCH_BOX || a.march == MARCH_BOX_HALO)  // LDS-staged box tiles, every epilogue form
     return box_apply_single(res_cb, psi_cb, gauge_cb, latt, prec, G, isign, tcb, lc.block,
-                            BoxEpi{a.xp, xa, norm_partial}, s);
+                            BoxEpi{a.xp, xa, norm_partial}, s, a.march == MARCH_BOX_HALO);
   if (nd == 4)
     launch_dslash_nd<4>(a, prec, G, isign, s);
   else
@@ -672,7 +678,7 @@ int dslash_set_launch(int nd, int prec, int gauge_reals, int block_threads, int 
 
 int dslash_set_launch_march(int nd, int prec, int gauge_reals, int march) {
   clear_error();
-  WD_REQUIRE(march_ok(march), "march must be 1, 4, 8 or 16 (got %d)", march);
+  WD_REQUIRE(march_ok(march), "march must be 1, 4, 8, 16 or 32 (got %d)", march);
   WD_REQUIRE(nd == 3 || nd == 4, "nd must be 3 or 4");
   if (int e = check_prec(prec)) return e;
   WD_REQUIRE(gauge_reals == 18 || gauge_reals == 12, "gauge_reals must be 18 or 12");

@@ -93,11 +93,16 @@ struct BoxEpi {
 // res = D psi (or an epilogue form) for one FP32, FP64 or sloppy-FP64 4D field
 // pair with box tiles and non-temporal links (the single-RHS path's march 16);
 // needs box_tiles_single(latt, block) > 0 and block >= 128.
+// halo: FP32 tiles whose y / z hops that leave the box read an LDS halo of the
+// neighbouring sites, loaded one slice ahead (needs box_halo_single(latt, block) > 0).
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
-                     int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s);
+                     int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s, bool halo = false);
 // Tiles (= blocks and |res|^2 partial sums) of that launch; 0 when Lx/2 > block,
 // when a box would hold fewer than 64 sites (the lanes would mostly idle) or when
 // the tiles would outnumber ceil(Vh / 64), the one-site kernel's most blocks.
 uint32_t box_tiles_single(const int latt[4], int block);
+// Halo entries of those tiles (one per thread at most, so 0 also when they would
+// exceed block, or when the box spans y and z or box tiles do not apply).
+uint32_t box_halo_single(const int latt[4], int block);
 
 }  // namespace wd

@@ -341,10 +341,52 @@ template <int P> __device__ inline Spinor<typename Fmt<P>::compute> box_spinor(c
   return q;
 }
 
-template <int P, int G, int IS, int R, int MU, bool NTL>
+// Halo of a one-field FP32 box (HALO tiles): the source sites just outside the
+// box in y and z (faces y0 - 1 and y0 + ty for every z of the box, z0 - 1 and
+// z0 + tz for every y; a face is absent when the box spans that direction),
+// each already projected for the one hop that uses it: (1 + isign gamma_mu) for
+// a backward face, (1 - isign gamma_mu) for a forward face.  Entries are
+// numbered [y backward | y forward | z backward | z forward], each face x-row
+// fastest; thread h of the block loads entry h.  LDS layout: [2 buffers][3
+// vectors][H] after the slab, buffer k & 1 holding slice t0 + k.
+struct BoxHalo {
+  v4u* buf;  // buffer 0; buffer 1 at buf + 3 H
+  int Hy;    // entries per y face (0: the box spans y)
+  int Hz;    // entries per z face (0: the box spans z)
+  int H;     // 2 Hy + 2 Hz
+};
+
+__device__ inline Half<float> half_from(const v4u* p, int stride) {
+  v4u buf[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) buf[v] = p[v * stride];
+  const float* r = reinterpret_cast<const float*>(buf);
+  Half<float> h;
+#pragma unroll
+  for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) h.h[sp][cc] = {r[(sp * 3 + cc) * 2], r[(sp * 3 + cc) * 2 + 1]};
+  return h;
+}
+
+__device__ inline void half_to(v4u* p, int stride, const Half<float>& h) {
+  v4u buf[3];
+  float* r = reinterpret_cast<float*>(buf);
+#pragma unroll
+  for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      r[(sp * 3 + cc) * 2] = h.h[sp][cc].re;
+      r[(sp * 3 + cc) * 2 + 1] = h.h[sp][cc].im;
+    }
+#pragma unroll
+  for (int v = 0; v < 3; ++v) p[v * stride] = buf[v];
+}
+
+template <int P, int G, int IS, int R, int MU, bool NTL, bool HALO = false>
 __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, const BoxLane& b, const Coord& c,
                                         uint32_t idx, uint32_t tS, int r0,
-                                        Spinor<typename Fmt<P>::compute> (&acc)[R]) {
+                                        Spinor<typename Fmt<P>::compute> (&acc)[R], const BoxHalo& hal = {}) {
   typedef typename Fmt<P>::compute T;
   constexpr int PV = PairVec<P, G>::value;
   constexpr int NV = 24 * Fmt<P>::bytes / 16;
@@ -373,6 +415,29 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
     lf = lane + (cf.z - b.z) * lxh * (int)L.ty;
     lb = lane + (cb.z - b.z) * lxh * (int)L.ty;
   }
+  if constexpr (HALO && MU != 0) {
+    static_assert(P == F32 && R == 1, "halo tiles take one FP32 field");
+    // in-box neighbours from the slab, out-of-box ones from the halo (both LDS):
+    // every lane reads both, at its own slot / entry 0 where unused, and keeps one
+    const int face = MU == 1 ? b.z - b.z0 : b.y - b.y0;
+    const int h0 = (MU == 1 ? 0 : 2 * hal.Hy) + face * lxh + b.i;
+    const int hs = MU == 1 ? hal.Hy : hal.Hz;
+    const Spinor<T> p = box_spinor<P>(slab + (inf ? lf : lane), nt);
+    const Spinor<T> q = box_spinor<P>(slab + (inb ? lb : lane), nt);
+    const Half<T> xf = half_from(hal.buf + (inf ? 0 : h0 + hs), hal.H);
+    const Half<T> xb = half_from(hal.buf + (inb ? 0 : h0), hal.H);
+    Half<T> hf = spin_project<MU, -IS>(p), hb = spin_project<MU, IS>(q);
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        hf.h[sp][cc] = inf ? hf.h[sp][cc] : xf.h[sp][cc];
+        hb.h[sp][cc] = inb ? hb.h[sp][cc] : xb.h[sp][cc];
+      }
+    spin_recon_add<MU, -IS>(acc[0], su3_mul(Wf, hf));
+    spin_recon_add<MU, IS>(acc[0], su3_mul(Wb, hb));
+    return;
+  }
   const uint32_t nf = site_index<4>(L.g, cf) - tS, nb = site_index<4>(L.g, cb) - tS;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -392,10 +457,28 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
 // non-temporally (one field: every link is used once), else cached for the
 // other RHS blocks of the same box.  EPI (one field only): EPI_XPAY stores
 // x + a D psi, EPI_XPAY_NORM also adds each of the lane's |res|^2 to s2.
-template <int P, int G, int IS, int R, bool NTL, int EPI>
+// One FP32 halo spinor (6 vectors in registers), projected for its face (mu = 1
+// or 2, backward or forward), to dst, dst + H, dst + 2 H.
+template <int IS>
+__device__ inline void box_halo_store(const v4u* raw, v4u* dst, int H, int hmu, int hfwd) {
+  const Spinor<float> q = box_spinor<F32>(raw, 1);
+  Half<float> h;
+  if (hmu == 1)
+    h = hfwd ? spin_project<1, -IS>(q) : spin_project<1, IS>(q);
+  else
+    h = hfwd ? spin_project<2, -IS>(q) : spin_project<2, IS>(q);
+  half_to(dst, H, h);
+}
+
+// HALO (one FP32 field): the y / z hops that leave the box read the projected
+// halo (BoxHalo) from LDS; thread h loads halo entry h of slice t + 1 while the
+// step for slice t runs, i.e. in the same step in which the neighbouring box
+// loads those sites for its own slab, and writes it into the other halo buffer.
+template <int P, int G, int IS, int R, bool NTL, int EPI, bool HALO = false>
 __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0,
                                               int r0, const BoxEpi& e, double& s2) {
   static_assert(EPI == EPI_PLAIN || R == 1, "the epilogue forms take one field");
+  static_assert(!HALO || (P == F32 && R == 1), "halo tiles take one FP32 field");
   typedef typename Fmt<P>::compute T;
   constexpr int TB = MRHS_BOX_SLICES;
   constexpr int PV = PairVec<P, G>::value;
@@ -415,6 +498,47 @@ __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, u
   }
   const uint32_t s = (uint32_t)b.i + (uint32_t)lxh * ((uint32_t)b.y + (uint32_t)L.g.L[1] * (uint32_t)b.z);
   const size_t bufsz = (size_t)R * NV * nt;
+  // this thread's halo entry: face (y or z, backward or forward) and source site
+  BoxHalo hal{};
+  bool hlive = false;
+  int hmu = 1, hfwd = 0;
+  uint32_t hsite = 0;
+  if constexpr (HALO) {
+    const int Ly = L.g.L[1], Lz = L.g.L[2], ty = (int)L.ty, tz = (int)L.tz;
+    hal.Hy = ty < Ly ? lxh * tz : 0;
+    hal.Hz = tz < Lz ? lxh * ty : 0;
+    hal.H = 2 * hal.Hy + 2 * hal.Hz;
+    hal.buf = slab + 2 * bufsz;
+    const int h = (int)threadIdx.x;
+    hlive = h < hal.H;
+    if (hlive) {
+      const int y0 = b.y0, z0 = b.z0;
+      int yy, zz, rem;
+      if (h < 2 * hal.Hy) {
+        hmu = 1;
+        hfwd = h >= hal.Hy;
+        rem = h - hfwd * hal.Hy;
+        yy = hfwd ? (y0 + ty == Ly ? 0 : y0 + ty) : (y0 == 0 ? Ly - 1 : y0 - 1);
+        zz = z0 + rem / lxh;
+      } else {
+        hmu = 2;
+        hfwd = h - 2 * hal.Hy >= hal.Hz;
+        rem = h - 2 * hal.Hy - hfwd * hal.Hz;
+        zz = hfwd ? (z0 + tz == Lz ? 0 : z0 + tz) : (z0 == 0 ? Lz - 1 : z0 - 1);
+        yy = y0 + rem / lxh;
+      }
+      hsite = (uint32_t)(rem % lxh) + (uint32_t)lxh * ((uint32_t)yy + (uint32_t)Ly * (uint32_t)zz);
+    }
+  }
+  if constexpr (HALO) {  // preamble: the halo of slice t0 -> halo buffer 0
+    if (hlive) {
+      const v4u* f = L.psi + (size_t)r0 * L.fstride + hsite + S * (uint32_t)t0;
+      v4u raw[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) raw[v] = f[(size_t)v * vh];
+      box_halo_store<IS>(raw, hal.buf + threadIdx.x, hal.H, hmu, hfwd);
+    }
+  }
   if (b.live) {  // preamble: slice t0 at the own site -> buffer 0, slice t0 - 1 -> buffer 1
     const uint32_t tm = (uint32_t)(t0 == 0 ? LT - 1 : t0 - 1);
 #pragma unroll
@@ -433,6 +557,16 @@ __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, u
     const int t = t0 + k;
     if (t >= LT) break;
     const int cur = k & 1;
+    // the next slice's halo entry, requested before this slice's work
+    const bool hnext = HALO && hlive && k + 1 < TB && t + 1 < LT;
+    v4u hraw[HALO ? NV : 1];
+    if constexpr (HALO) {
+      if (hnext) {
+        const v4u* f = L.psi + (size_t)r0 * L.fstride + hsite + S * (uint32_t)(t + 1);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) hraw[v] = f[(size_t)v * vh];
+      }
+    }
     if (b.live) {
       const Coord c{2 * b.i + ((L.tcb + b.y + b.z + t) & 1), b.y, b.z, t};
       const uint32_t tS = S * (uint32_t)t;
@@ -472,12 +606,14 @@ __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, u
       // spills fenced) and the kernel is 2.4% faster at 32^3 x 64; for FP32 and
       // sloppy FP64 the difference is within 0.5%.
       constexpr bool FENCE = !(NTL && P == F64);
+      BoxHalo hc = hal;
+      if constexpr (HALO) hc.buf = hal.buf + (size_t)cur * 3 * hal.H;
       if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
-      box_hop<P, G, IS, R, 0, NTL>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<P, G, IS, R, 0, NTL, HALO>(L, here, b, c, idx, tS, r0, acc, hc);
       if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
-      box_hop<P, G, IS, R, 1, NTL>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<P, G, IS, R, 1, NTL, HALO>(L, here, b, c, idx, tS, r0, acc, hc);
       if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
-      box_hop<P, G, IS, R, 2, NTL>(L, here, b, c, idx, tS, r0, acc);
+      box_hop<P, G, IS, R, 2, NTL, HALO>(L, here, b, c, idx, tS, r0, acc, hc);
       if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
       if constexpr (EPI == EPI_PLAIN) {
 #pragma unroll
@@ -498,6 +634,9 @@ __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, u
         if constexpr (EPI == EPI_XPAY_NORM) s2 += (double)n2;
         store_spinor<P, true>(L.res, nullptr, idx, vh, acc[0]);
       }
+    }
+    if constexpr (HALO) {
+      if (hnext) box_halo_store<IS>(hraw, hal.buf + (size_t)(cur ^ 1) * 3 * hal.H + threadIdx.x, hal.H, hmu, hfwd);
     }
     __syncthreads();  // buffer cur ^ 1 complete; buffer cur free for the next step's writes
   }
@@ -521,7 +660,7 @@ __device__ inline void box_block_sum(double v, double* out) {
 }
 
 // The box-tile grid: each block walks its XCD's contiguous share of the tiles.
-template <int P, int G, int R, int IS, bool NTL, int EPI>
+template <int P, int G, int R, int IS, bool NTL, int EPI, bool HALO = false>
 __device__ __forceinline__ void mrhs_box_grid(const MrhsList& list, const BoxEpi& e) {
   extern __shared__ v4u box_slab[];
   const uint32_t NX = gridDim.x < 8 ? gridDim.x : 8;
@@ -542,8 +681,8 @@ __device__ __forceinline__ void mrhs_box_grid(const MrhsList& list, const BoxEpi
     const uint32_t ng = L.nrb / L.rg, tc = q2 / ng, grp = q2 - tc * ng;
     const uint32_t btz = box / L.nty, bty = box - btz * L.nty;
     double s2 = 0.0;
-    mrhs_box_tile<P, G, IS, R, NTL, EPI>(L, box_slab, bty, btz, (int)tc * MRHS_BOX_SLICES,
-                                         (int)(grp * L.rg + rbi) * R, e, s2);
+    mrhs_box_tile<P, G, IS, R, NTL, EPI, HALO>(L, box_slab, bty, btz, (int)tc * MRHS_BOX_SLICES,
+                                               (int)(grp * L.rg + rbi) * R, e, s2);
     if constexpr (EPI == EPI_XPAY_NORM) box_block_sum(s2, e.norm_partial + 2 * (size_t)t);
   }
 }
@@ -551,6 +690,12 @@ __device__ __forceinline__ void mrhs_box_grid(const MrhsList& list, const BoxEpi
 template <int P, int G, int R, int IS, bool NTL, int EPI = EPI_PLAIN>
 __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list, BoxEpi e) {
   mrhs_box_grid<P, G, R, IS, NTL, EPI>(list, e);
+}
+
+// One FP32 field with the LDS halo for the y / z hops that leave the box.
+template <int G, int IS, int EPI>
+__global__ void __launch_bounds__(256) k_box_halo(MrhsList list, BoxEpi e) {
+  mrhs_box_grid<F32, G, 1, IS, true, EPI, true>(list, e);
 }
 
 // Wide FP32 tiles: R x blockDim.x <= MRHS_BOX_WIDE (one block per CU), cached links.
@@ -747,10 +892,39 @@ uint32_t box_tiles_single(const int latt[4], int block) {
   return tiles <= (vh + 63) / 64 ? (uint32_t)tiles : 0;
 }
 
+// Halo entries of a box of ty x tz rows and planes (0 when the box spans y and z).
+static int box_halo_entries(int lxh, int ly, int lz, int ty, int tz) {
+  const int hy = ty < ly ? lxh * tz : 0;
+  const int hz = tz < lz ? lxh * ty : 0;
+  return 2 * hy + 2 * hz;
+}
+
+uint32_t box_halo_single(const int latt[4], int block) {
+  if (box_tiles_single(latt, block) == 0) return 0;
+  uint32_t ty, tz;
+  box_shape(latt[0] / 2, latt[1], latt[2], block, &ty, &tz);
+  const int H = box_halo_entries(latt[0] / 2, latt[1], latt[2], (int)ty, (int)tz);
+  return H <= block ? (uint32_t)H : 0;
+}
+
 template <int P, int G, int IS>
-static void launch_box_single_s(const MrhsList& list, int block, const BoxEpi& e, hipStream_t s) {
+static void launch_box_single_s(const MrhsList& list, int block, const BoxEpi& e, hipStream_t s, bool halo) {
   const size_t lds = (size_t)2 * spinor_vec(P) * block * sizeof(v4u);
   const dim3 grid(list.total_tiles), blk(block);
+  if constexpr (P == F32) {
+    const MrhsLattice& L = list.lat[0];
+    const int H = box_halo_entries(L.g.lxh, L.g.L[1], L.g.L[2], (int)L.ty, (int)L.tz);
+    if (halo && H >= 1 && H <= block) {  // one halo entry per thread at most
+      const size_t lh = lds + (size_t)2 * 3 * H * sizeof(v4u);
+      if (!e.xp)
+        hipLaunchKernelGGL((k_box_halo<G, IS, EPI_PLAIN>), grid, blk, lh, s, list, e);
+      else if (!e.norm_partial)
+        hipLaunchKernelGGL((k_box_halo<G, IS, EPI_XPAY>), grid, blk, lh, s, list, e);
+      else
+        hipLaunchKernelGGL((k_box_halo<G, IS, EPI_XPAY_NORM>), grid, blk, lh, s, list, e);
+      return;
+    }
+  }
   if (!e.xp)
     hipLaunchKernelGGL((k_mrhs_box<P, G, 1, IS, true, EPI_PLAIN>), grid, blk, lds, s, list, e);
   else if (!e.norm_partial)
@@ -760,15 +934,16 @@ static void launch_box_single_s(const MrhsList& list, int block, const BoxEpi& e
 }
 
 template <int P, int G>
-static void launch_box_single(const MrhsList& list, int isign, int block, const BoxEpi& e, hipStream_t s) {
+static void launch_box_single(const MrhsList& list, int isign, int block, const BoxEpi& e, hipStream_t s,
+                              bool halo) {
   if (isign > 0)
-    launch_box_single_s<P, G, 1>(list, block, e, s);
+    launch_box_single_s<P, G, 1>(list, block, e, s, halo);
   else
-    launch_box_single_s<P, G, -1>(list, block, e, s);
+    launch_box_single_s<P, G, -1>(list, block, e, s, halo);
 }
 
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
-                     int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s) {
+                     int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s, bool halo) {
   WD_REQUIRE(prec == F32 || prec == F64 || prec == F64S, "box tiles are built for F32, F64 and F64S");
   WD_REQUIRE(block >= 128 && box_tiles_single(latt, block) > 0,
              "box tiles need >= 128 threads, Lx/2 <= block_threads and >= 64 sites per box");
@@ -785,14 +960,14 @@ int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, con
   MrhsList list;
   if (int e = mrhs_build_list(&p, 1, prec, 1, block, MRHS_T_BOX, &list)) return e;
   if (prec == F32) {
-    if (G == 18) launch_box_single<F32, 18>(list, isign, block, epi, s);
-    else launch_box_single<F32, 12>(list, isign, block, epi, s);
+    if (G == 18) launch_box_single<F32, 18>(list, isign, block, epi, s, halo);
+    else launch_box_single<F32, 12>(list, isign, block, epi, s, halo);
   } else if (prec == F64) {
-    if (G == 18) launch_box_single<F64, 18>(list, isign, block, epi, s);
-    else launch_box_single<F64, 12>(list, isign, block, epi, s);
+    if (G == 18) launch_box_single<F64, 18>(list, isign, block, epi, s, false);
+    else launch_box_single<F64, 12>(list, isign, block, epi, s, false);
   } else {
-    if (G == 18) launch_box_single<F64S, 18>(list, isign, block, epi, s);
-    else launch_box_single<F64S, 12>(list, isign, block, epi, s);
+    if (G == 18) launch_box_single<F64S, 18>(list, isign, block, epi, s, false);
+    else launch_box_single<F64S, 12>(list, isign, block, epi, s, false);
   }
   WD_LAUNCHED();
   return 0;

@@ -137,9 +137,10 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   // Candidates are timed in three interleaved rounds and scored by their best
   // round, so a transient slowdown of the device does not pick a shape.
   const int blocks[4] = {64, 128, 192, 256};
-  const int marches[4] = {1, 4, 8, 16};
-  const int nm = (nd == 4 && (prec == F32 || prec == F64 || prec == F64S)) ? 4 : 3;  // box tiles (16): 4D
-  float tmin[4][2][4];
+  const int marches[5] = {1, 4, 8, MARCH_BOX, MARCH_BOX_HALO};
+  // box tiles (16): 4D FP32, FP64, sloppy FP64; with the LDS halo (32): 4D FP32
+  const int nm = nd == 4 && prec == F32 ? 5 : (nd == 4 && (prec == F64 || prec == F64S)) ? 4 : 3;
+  float tmin[4][2][5];
   for (auto& a : tmin)
     for (auto& b : a)
       for (float& t : b) t = 1e30f;
@@ -151,7 +152,7 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
           // construction); tuning them with remap 1 only keeps the flag right for
           // the one-site kernels that share this entry (the domain-decomposition
           // interior, lattices below the box thread floor)
-          if (marches[mi] == 16 && (blocks[bi] < 128 || remap == 0)) continue;
+          if (marches[mi] >= MARCH_BOX && (blocks[bi] < 128 || remap == 0)) continue;
           set_launch_cfg(nd, prec, G, LaunchCfg{blocks[bi], remap, marches[mi]});
           float t = 0;
           int k = 0;

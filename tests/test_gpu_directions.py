@@ -143,11 +143,16 @@ class Shape:
 
 CASES = [
     # (nd, prec, G, march, lattice): march 1 one site per thread, 4/8 marching,
-    # 16 LDS-staged box tiles; 32^3 x 64 is large enough for every shape to run
+    # 16 LDS-staged box tiles, 32 box tiles with the LDS halo (at 24 x 48^3 boxes of
+    # 12 x 3 x 6 sites, halo faces of unequal size); 32^3 x 64 is large enough for
+    # every shape to run
     (4, F32, 18, 1, (32, 32, 32, 64)),
     (4, F32, 18, 8, (32, 32, 32, 64)),
     (4, F32, 18, 16, (32, 32, 32, 64)),
     (4, F32, 12, 16, (32, 32, 32, 64)),
+    (4, F32, 18, 32, (32, 32, 32, 64)),
+    (4, F32, 12, 32, (32, 32, 32, 64)),
+    (4, F32, 18, 32, (24, 48, 48, 48)),
     (4, F64, 18, 16, (32, 32, 32, 64)),
     (4, F64, 18, 4, (32, 32, 32, 64)),
     (4, F64S, 18, 16, (32, 32, 32, 64)),

@@ -311,7 +311,8 @@ def test_marching_lengths(oracle, nd, latt, prec, G):
     lib = _lib.core()
     b0, r0 = C.c_int(), C.c_int()
     _lib.check(lib.dslash_get_launch(nd, prec, G, C.byref(b0), C.byref(r0)))
-    marches = (1, 4, 8, 16) if (nd == 4 and prec in (F32, F64, F64S)) else (1, 4, 8)
+    marches = (1, 4, 8, 16, 32) if (nd == 4 and prec == F32) else \
+        (1, 4, 8, 16) if (nd == 4 and prec in (F64, F64S)) else (1, 4, 8)
     with _March(nd, prec, G) as m:
         for isign, cb in ((1, 0), (-1, 1)):
             want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
@@ -320,7 +321,7 @@ def test_marching_lengths(oracle, nd, latt, prec, G):
             first = None
             for march in marches:
                 m.set(march)
-                _lib.check(lib.dslash_set_launch(nd, prec, G, 256 if march == 16 else b0.value, r0.value))
+                _lib.check(lib.dslash_set_launch(nd, prec, G, 256 if march >= 16 else b0.value, r0.value))
                 assert _march_for(latt, nd, prec, G) == march
                 d_res = op.pack_spinor(np.full_like(psi, 3.25))
                 op.apply(d_res, d_psi, isign, cb)
@@ -384,9 +385,9 @@ XPAY_BOX_CASES = [((32, 32, 32, 66), F64, 18), ((24, 48, 48, 48), F32, 18), ((64
 @pytest.mark.parametrize("latt,prec,G", XPAY_BOX_CASES)
 def test_xpay_epilogue_box_tiles(oracle, latt, prec, G):
     """dslash_apply_xpay on lattices where the 4D operator runs LDS-staged box
-    tiles (march 16): res = x + a D psi agrees with the oracle and with the
-    one-site-per-thread kernel (march 1), and the in-place form (x aliasing res)
-    gives the same bits as the out-of-place one."""
+    tiles (march 16; for FP32 also with the LDS halo, march 32): res = x + a D psi
+    agrees with the oracle and with the one-site-per-thread kernel (march 1), and
+    the in-place form (x aliasing res) gives the same bits as the out-of-place one."""
     import ctypes as C
     from sse_wilson_dslash_amd import _lib
     from sse_wilson_dslash_amd._lib import ptr
@@ -412,7 +413,7 @@ def test_xpay_epilogue_box_tiles(oracle, latt, prec, G):
                 t = slice(tcb * vh, (tcb + 1) * vh)
                 want = (x + a * oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, 1 - tcb)).reshape(-1, 24)[t]
                 outs = {}
-                for march in (16, 1):
+                for march in ((32, 16, 1) if prec == F32 else (16, 1)):
                     m.set(march)
                     _lib.check(lib.dslash_set_launch(4, prec, G, 256, 1))
                     assert _march_for(latt, 4, prec, G) == march
@@ -430,7 +431,8 @@ def test_xpay_epilogue_box_tiles(oracle, latt, prec, G):
                     assert e2 < MARCH_TOL[prec] and einf < MARCH_TOL[prec], (march, isign, tcb, e2, einf)
                     assert np.array_equal(got_in, got), (march, isign, tcb)
                     outs[march] = got
-                assert np.max(np.abs(outs[16] - outs[1])) <= MARCH_SAME[prec] * np.max(np.abs(outs[1]))
+                for mb in outs:
+                    assert np.max(np.abs(outs[mb] - outs[1])) <= MARCH_SAME[prec] * np.max(np.abs(outs[1])), mb
     finally:
         _lib.check(lib.dslash_set_launch(4, prec, G, b0.value, r0.value))
     del op

@@ -162,8 +162,8 @@ def test_march_setting():
     m = C.c_int()
     assert lib.dslash_get_launch_march(3, _lib.F32, 18, C.byref(m)) == 0
     saved = m.value
-    assert saved in (1, 4, 8, 16)
-    for bad in (0, 2, 3, 32, -1):
+    assert saved in (1, 4, 8, 16, 32)
+    for bad in (0, 2, 3, 24, 64, -1):
         assert lib.dslash_set_launch_march(3, _lib.F32, 18, bad) == _lib.ERR_INVALID
     assert b"march" in lib.dslash_last_error_message()
     assert lib.dslash_set_launch_march(5, _lib.F32, 18, 4) == _lib.ERR_INVALID
@@ -244,6 +244,25 @@ def test_box_tiles_single_rhs_fallbacks():
         assert eff((600, 8, 8, 64)) == 8     # Lx/2 = 300 > 256 threads
         assert lib.dslash_set_launch(4, _lib.F32, 18, 64, 1) == 0
         assert eff((32, 32, 32, 64)) == 8    # 64 threads per block: no box tiles
+        # 32: box tiles with the LDS halo, where the halo takes at most one entry per thread
+        assert lib.dslash_set_launch_march(4, _lib.F32, 18, 32) == 0
+        assert lib.dslash_set_launch(4, _lib.F32, 18, 256, 1) == 0
+        assert eff((32, 32, 32, 64)) == 32   # 16 x 4 x 4 boxes: 2 x 64 y + 2 x 64 z halo sites = 256
+        assert eff((32, 32, 32, 66)) == 32
+        assert eff((48, 48, 48, 96)) == 16   # 24 x 3 x 3 boxes: 288 halo sites > 256 threads
+        assert eff((8, 8, 8, 8)) == 1        # as for 16
+        assert eff((600, 8, 8, 64)) == 8
+        assert lib.dslash_set_launch(4, _lib.F32, 18, 64, 1) == 0
+        assert eff((32, 32, 32, 64)) == 8
+        lib.dslash_get_launch(4, _lib.F64, 18, C.byref(b), C.byref(r))
+        lib.dslash_get_launch_march(4, _lib.F64, 18, C.byref(m))
+        f64 = (b.value, r.value, m.value)
+        assert lib.dslash_set_launch_march(4, _lib.F64, 18, 32) == 0  # FP64: the plain box tiles
+        assert lib.dslash_set_launch(4, _lib.F64, 18, 256, 1) == 0
+        assert lib.dslash_launch_march_for(_lib.latt_arg((32, 32, 32, 64)), 4, _lib.F64, 18, C.byref(m)) == 0
+        assert m.value == 16
+        assert lib.dslash_set_launch(4, _lib.F64, 18, f64[0], f64[1]) == 0
+        assert lib.dslash_set_launch_march(4, _lib.F64, 18, f64[2]) == 0
     finally:
         lib.dslash_set_launch(4, _lib.F32, 18, saved[0], saved[1])
         lib.dslash_set_launch_march(4, _lib.F32, 18, saved[2])

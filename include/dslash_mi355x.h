@@ -152,7 +152,7 @@ int dslash_get_launch(int nd, int prec, int gauge_reals, int* block_threads, int
  * (>= 128, >= Lx/2) holds whole x rows of a ty x tz box of (y, z), walks 8
  * t-slices, and stages each slice's source spinors in LDS for the x hops and
  * the y / z hops inside the box (links non-temporal); 32 = the same box tiles
- * (4D FP32) whose y / z hops that leave the box read an LDS halo: each thread
+ * (4D FP32, sloppy FP64) whose y / z hops that leave the box read an LDS halo: each thread
  * loads one neighbouring site of the next slice while the current one is
  * computed and stores it projected for its hop.  Any lattice extent is
  * allowed (the last chunk is shorter).  A launch that would have fewer than

@@ -13,7 +13,7 @@ struct LaunchCfg {
   int march;  // slices of the slowest coordinate per thread (1, 4 or 8; the x + a D psi
               // forms take 1), 16 = LDS-staged box tiles (4D FP32, FP64, sloppy
               // FP64; every form) or 32 = box tiles whose out-of-box y / z hops read
-              // an LDS halo (4D FP32; else 16)
+              // an LDS halo (4D FP32 and sloppy FP64; else 16)
 };
 constexpr int MARCH_BOX = 16, MARCH_BOX_HALO = 32;
 inline bool march_ok(int m) { return m == 1 || m == 4 || m == 8 || m == MARCH_BOX || m == MARCH_BOX_HALO; }

@@ -63,7 +63,7 @@ void clear_error() {
 // 1% at 48^3 x 96, where marching is 3-5% slower, and box tiles with the LDS halo
 // (march 32) another 5% (18-real links) and 9% (12-real) faster at 32^3 x 64;
 // for 4D FP64 box tiles are 3-4% and for sloppy FP64 6-10% faster than a march
-// of 8).
+// of 8, and sloppy FP64 with the halo another 4.6% (18-real) / 1.4% (12-real)).
 // Index: [nd == 4][F32, F64, F16, BF16, F64S][G == 12].
 static LaunchCfg g_cfg[2][NUM_PREC][2] = {
     {{{256, 1, 8}, {256, 1, 8}},
@@ -75,7 +75,7 @@ static LaunchCfg g_cfg[2][NUM_PREC][2] = {
      {{256, 1, 16}, {256, 1, 16}},
      {{256, 1, 1}, {256, 1, 1}},
      {{256, 1, 1}, {256, 1, 1}},
-     {{256, 1, 16}, {256, 1, 16}}}};
+     {{256, 1, 32}, {256, 1, 32}}}};
 
 static int prec_slot(int prec) { return prec >= 0 && prec < NUM_PREC ? prec : 0; }
 LaunchCfg launch_cfg(int nd, int prec, int G) { return g_cfg[nd == 4][prec_slot(prec)][G == 12]; }
@@ -343,13 +343,15 @@ static void launch_dslash_nd(const ApplyArgs& a, int prec, int G, int isign, hip
 // MARCH_MIN_THREADS threads.
 // Box tiles (16) need the 4D FP32, FP64 or sloppy-FP64 operator, a whole x row per block and
 // the same thread floor; otherwise they fall back to a march of 8.  Box tiles with
-// the LDS halo (32) need FP32 and a halo of at most one entry per thread; otherwise
-// they fall back to 16.
+// the LDS halo (32) need FP32 or sloppy FP64 (FP64 would spill: its tile already
+// takes 256 VGPRs) and a halo of at most one entry per thread; otherwise they fall
+// back to 16.
 static int effective_march(const int latt[4], int nd, int prec, int block, int march) {
   const uint64_t L = (uint64_t)latt[nd - 1];
   const uint64_t slice = (uint64_t)latt[0] * latt[1] * latt[2] * latt[3] / 2 / L;
   int m = march_ok(march) ? march : 1;
-  if (m == MARCH_BOX_HALO) m = (prec == F32 && box_halo_single(latt, block) > 0) ? MARCH_BOX_HALO : MARCH_BOX;
+  if (m == MARCH_BOX_HALO)
+    m = (prec == F32 || prec == F64S) && box_halo_single(latt, block) > 0 ? MARCH_BOX_HALO : MARCH_BOX;
   if (m == MARCH_BOX || m == MARCH_BOX_HALO) {
     if (nd == 4 && (prec == F32 || prec == F64 || prec == F64S) && block >= 128 &&
         (uint64_t)box_tiles_single(latt, block) * (uint64_t)block >= MARCH_MIN_THREADS)

@@ -341,27 +341,32 @@ template <int P> __device__ inline Spinor<typename Fmt<P>::compute> box_spinor(c
   return q;
 }
 
-// Halo of a one-field FP32 box (HALO tiles): the source sites just outside the
+// Halo of a one-field box (HALO tiles): the source sites just outside the
 // box in y and z (faces y0 - 1 and y0 + ty for every z of the box, z0 - 1 and
 // z0 + tz for every y; a face is absent when the box spans that direction),
 // each already projected for the one hop that uses it: (1 + isign gamma_mu) for
 // a backward face, (1 - isign gamma_mu) for a forward face.  Entries are
 // numbered [y backward | y forward | z backward | z forward], each face x-row
-// fastest; thread h of the block loads entry h.  LDS layout: [2 buffers][3
-// vectors][H] after the slab, buffer k & 1 holding slice t0 + k.
+// fastest; thread h of the block loads entry h.  Entries hold the 12 reals in
+// the compute type (3 vectors for FP32 and sloppy FP64, 6 for FP64).  LDS layout:
+// [2 buffers][HV vectors][H] after the slab, buffer k & 1 holding slice t0 + k.
 struct BoxHalo {
-  v4u* buf;  // buffer 0; buffer 1 at buf + 3 H
+  v4u* buf;  // buffer 0; buffer 1 at buf + HV H
   int Hy;    // entries per y face (0: the box spans y)
   int Hz;    // entries per z face (0: the box spans z)
   int H;     // 2 Hy + 2 Hz
 };
 
-__device__ inline Half<float> half_from(const v4u* p, int stride) {
-  v4u buf[3];
+// A half spinor as 12 reals of T in 3 (float) or 6 (double) 16-byte vectors at
+// p, p + stride, ...
+template <typename T> constexpr int half_vecs() { return 12 * (int)sizeof(T) / 16; }
+template <typename T> __device__ inline Half<T> half_from(const v4u* p, int stride) {
+  constexpr int HV = half_vecs<T>();
+  v4u buf[HV];
 #pragma unroll
-  for (int v = 0; v < 3; ++v) buf[v] = p[v * stride];
-  const float* r = reinterpret_cast<const float*>(buf);
-  Half<float> h;
+  for (int v = 0; v < HV; ++v) buf[v] = p[v * stride];
+  const T* r = reinterpret_cast<const T*>(buf);
+  Half<T> h;
 #pragma unroll
   for (int sp = 0; sp < 2; ++sp)
 #pragma unroll
@@ -369,9 +374,10 @@ __device__ inline Half<float> half_from(const v4u* p, int stride) {
   return h;
 }
 
-__device__ inline void half_to(v4u* p, int stride, const Half<float>& h) {
-  v4u buf[3];
-  float* r = reinterpret_cast<float*>(buf);
+template <typename T> __device__ inline void half_to(v4u* p, int stride, const Half<T>& h) {
+  constexpr int HV = half_vecs<T>();
+  v4u buf[HV];
+  T* r = reinterpret_cast<T*>(buf);
 #pragma unroll
   for (int sp = 0; sp < 2; ++sp)
 #pragma unroll
@@ -380,7 +386,7 @@ __device__ inline void half_to(v4u* p, int stride, const Half<float>& h) {
       r[(sp * 3 + cc) * 2 + 1] = h.h[sp][cc].im;
     }
 #pragma unroll
-  for (int v = 0; v < 3; ++v) p[v * stride] = buf[v];
+  for (int v = 0; v < HV; ++v) p[v * stride] = buf[v];
 }
 
 template <int P, int G, int IS, int R, int MU, bool NTL, bool HALO = false>
@@ -416,7 +422,7 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
     lb = lane + (cb.z - b.z) * lxh * (int)L.ty;
   }
   if constexpr (HALO && MU != 0) {
-    static_assert(P == F32 && R == 1, "halo tiles take one FP32 field");
+    static_assert(R == 1, "halo tiles take one field");
     // in-box neighbours from the slab, out-of-box ones from the halo (both LDS):
     // every lane reads both, at its own slot / entry 0 where unused, and keeps one
     const int face = MU == 1 ? b.z - b.z0 : b.y - b.y0;
@@ -424,8 +430,8 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
     const int hs = MU == 1 ? hal.Hy : hal.Hz;
     const Spinor<T> p = box_spinor<P>(slab + (inf ? lf : lane), nt);
     const Spinor<T> q = box_spinor<P>(slab + (inb ? lb : lane), nt);
-    const Half<T> xf = half_from(hal.buf + (inf ? 0 : h0 + hs), hal.H);
-    const Half<T> xb = half_from(hal.buf + (inb ? 0 : h0), hal.H);
+    const Half<T> xf = half_from<T>(hal.buf + (inf ? 0 : h0 + hs), hal.H);
+    const Half<T> xb = half_from<T>(hal.buf + (inb ? 0 : h0), hal.H);
     Half<T> hf = spin_project<MU, -IS>(p), hb = spin_project<MU, IS>(q);
 #pragma unroll
     for (int sp = 0; sp < 2; ++sp)
@@ -457,12 +463,13 @@ __device__ __forceinline__ void box_hop(const MrhsLattice& L, const v4u* slab, c
 // non-temporally (one field: every link is used once), else cached for the
 // other RHS blocks of the same box.  EPI (one field only): EPI_XPAY stores
 // x + a D psi, EPI_XPAY_NORM also adds each of the lane's |res|^2 to s2.
-// One FP32 halo spinor (6 vectors in registers), projected for its face (mu = 1
-// or 2, backward or forward), to dst, dst + H, dst + 2 H.
-template <int IS>
+// One halo spinor of format P (its vectors in registers), projected for its face
+// (mu = 1 or 2, backward or forward) in the compute type, to dst, dst + H, ...
+template <int P, int IS>
 __device__ inline void box_halo_store(const v4u* raw, v4u* dst, int H, int hmu, int hfwd) {
-  const Spinor<float> q = box_spinor<F32>(raw, 1);
-  Half<float> h;
+  typedef typename Fmt<P>::compute T;
+  const Spinor<T> q = box_spinor<P>(raw, 1);
+  Half<T> h;
   if (hmu == 1)
     h = hfwd ? spin_project<1, -IS>(q) : spin_project<1, IS>(q);
   else
@@ -470,7 +477,7 @@ __device__ inline void box_halo_store(const v4u* raw, v4u* dst, int H, int hmu, 
   half_to(dst, H, h);
 }
 
-// HALO (one FP32 field): the y / z hops that leave the box read the projected
+// HALO (one field): the y / z hops that leave the box read the projected
 // halo (BoxHalo) from LDS; thread h loads halo entry h of slice t + 1 while the
 // step for slice t runs, i.e. in the same step in which the neighbouring box
 // loads those sites for its own slab, and writes it into the other halo buffer.
@@ -478,8 +485,9 @@ template <int P, int G, int IS, int R, bool NTL, int EPI, bool HALO = false>
 __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, uint32_t bty, uint32_t btz, int t0,
                                               int r0, const BoxEpi& e, double& s2) {
   static_assert(EPI == EPI_PLAIN || R == 1, "the epilogue forms take one field");
-  static_assert(!HALO || (P == F32 && R == 1), "halo tiles take one FP32 field");
+  static_assert(!HALO || R == 1, "halo tiles take one field");
   typedef typename Fmt<P>::compute T;
+  constexpr int HV = half_vecs<T>();  // vectors per halo entry
   constexpr int TB = MRHS_BOX_SLICES;
   constexpr int PV = PairVec<P, G>::value;
   constexpr int NV = 24 * Fmt<P>::bytes / 16;
@@ -536,7 +544,7 @@ __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, u
       v4u raw[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) raw[v] = f[(size_t)v * vh];
-      box_halo_store<IS>(raw, hal.buf + threadIdx.x, hal.H, hmu, hfwd);
+      box_halo_store<P, IS>(raw, hal.buf + threadIdx.x, hal.H, hmu, hfwd);
     }
   }
   if (b.live) {  // preamble: slice t0 at the own site -> buffer 0, slice t0 - 1 -> buffer 1
@@ -607,7 +615,7 @@ __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, u
       // sloppy FP64 the difference is within 0.5%.
       constexpr bool FENCE = !(NTL && P == F64);
       BoxHalo hc = hal;
-      if constexpr (HALO) hc.buf = hal.buf + (size_t)cur * 3 * hal.H;
+      if constexpr (HALO) hc.buf = hal.buf + (size_t)cur * HV * hal.H;
       if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
       box_hop<P, G, IS, R, 0, NTL, HALO>(L, here, b, c, idx, tS, r0, acc, hc);
       if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
@@ -636,7 +644,7 @@ __device__ __forceinline__ void mrhs_box_tile(const MrhsLattice& L, v4u* slab, u
       }
     }
     if constexpr (HALO) {
-      if (hnext) box_halo_store<IS>(hraw, hal.buf + (size_t)(cur ^ 1) * 3 * hal.H + threadIdx.x, hal.H, hmu, hfwd);
+      if (hnext) box_halo_store<P, IS>(hraw, hal.buf + (size_t)(cur ^ 1) * HV * hal.H + threadIdx.x, hal.H, hmu, hfwd);
     }
     __syncthreads();  // buffer cur ^ 1 complete; buffer cur free for the next step's writes
   }
@@ -696,6 +704,12 @@ __global__ void __launch_bounds__(256) k_mrhs_box(MrhsList list, BoxEpi e) {
 template <int G, int IS, int EPI>
 __global__ void __launch_bounds__(256) k_box_halo(MrhsList list, BoxEpi e) {
   mrhs_box_grid<F32, G, 1, IS, true, EPI, true>(list, e);
+}
+
+// The same for one FP64 or sloppy-FP64 field (halo entries in the compute type).
+template <int P, int G, int IS, int EPI>
+__global__ void __launch_bounds__(256) k_box_halo64(MrhsList list, BoxEpi e) {
+  mrhs_box_grid<P, G, 1, IS, true, EPI, true>(list, e);
 }
 
 // Wide FP32 tiles: R x blockDim.x <= MRHS_BOX_WIDE (one block per CU), cached links.
@@ -911,17 +925,26 @@ template <int P, int G, int IS>
 static void launch_box_single_s(const MrhsList& list, int block, const BoxEpi& e, hipStream_t s, bool halo) {
   const size_t lds = (size_t)2 * spinor_vec(P) * block * sizeof(v4u);
   const dim3 grid(list.total_tiles), blk(block);
-  if constexpr (P == F32) {
+  {
     const MrhsLattice& L = list.lat[0];
     const int H = box_halo_entries(L.g.lxh, L.g.L[1], L.g.L[2], (int)L.ty, (int)L.tz);
     if (halo && H >= 1 && H <= block) {  // one halo entry per thread at most
-      const size_t lh = lds + (size_t)2 * 3 * H * sizeof(v4u);
-      if (!e.xp)
-        hipLaunchKernelGGL((k_box_halo<G, IS, EPI_PLAIN>), grid, blk, lh, s, list, e);
-      else if (!e.norm_partial)
-        hipLaunchKernelGGL((k_box_halo<G, IS, EPI_XPAY>), grid, blk, lh, s, list, e);
-      else
-        hipLaunchKernelGGL((k_box_halo<G, IS, EPI_XPAY_NORM>), grid, blk, lh, s, list, e);
+      const size_t lh = lds + (size_t)2 * half_vecs<typename Fmt<P>::compute>() * H * sizeof(v4u);
+      if constexpr (P == F32) {
+        if (!e.xp)
+          hipLaunchKernelGGL((k_box_halo<G, IS, EPI_PLAIN>), grid, blk, lh, s, list, e);
+        else if (!e.norm_partial)
+          hipLaunchKernelGGL((k_box_halo<G, IS, EPI_XPAY>), grid, blk, lh, s, list, e);
+        else
+          hipLaunchKernelGGL((k_box_halo<G, IS, EPI_XPAY_NORM>), grid, blk, lh, s, list, e);
+      } else {
+        if (!e.xp)
+          hipLaunchKernelGGL((k_box_halo64<P, G, IS, EPI_PLAIN>), grid, blk, lh, s, list, e);
+        else if (!e.norm_partial)
+          hipLaunchKernelGGL((k_box_halo64<P, G, IS, EPI_XPAY>), grid, blk, lh, s, list, e);
+        else
+          hipLaunchKernelGGL((k_box_halo64<P, G, IS, EPI_XPAY_NORM>), grid, blk, lh, s, list, e);
+      }
       return;
     }
   }
@@ -963,11 +986,11 @@ int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, con
     if (G == 18) launch_box_single<F32, 18>(list, isign, block, epi, s, halo);
     else launch_box_single<F32, 12>(list, isign, block, epi, s, halo);
   } else if (prec == F64) {
-    if (G == 18) launch_box_single<F64, 18>(list, isign, block, epi, s, false);
-    else launch_box_single<F64, 12>(list, isign, block, epi, s, false);
+    if (G == 18) launch_box_single<F64, 18>(list, isign, block, epi, s, halo);
+    else launch_box_single<F64, 12>(list, isign, block, epi, s, halo);
   } else {
-    if (G == 18) launch_box_single<F64S, 18>(list, isign, block, epi, s, false);
-    else launch_box_single<F64S, 12>(list, isign, block, epi, s, false);
+    if (G == 18) launch_box_single<F64S, 18>(list, isign, block, epi, s, halo);
+    else launch_box_single<F64S, 12>(list, isign, block, epi, s, halo);
   }
   WD_LAUNCHED();
   return 0;

@@ -138,8 +138,8 @@ int dslash_tune_apply(void* d_res, const void* d_psi, const void* d_gauge, const
   // round, so a transient slowdown of the device does not pick a shape.
   const int blocks[4] = {64, 128, 192, 256};
   const int marches[5] = {1, 4, 8, MARCH_BOX, MARCH_BOX_HALO};
-  // box tiles (16): 4D FP32, FP64, sloppy FP64; with the LDS halo (32): 4D FP32
-  const int nm = nd == 4 && prec == F32 ? 5 : (nd == 4 && (prec == F64 || prec == F64S)) ? 4 : 3;
+  // box tiles (16): 4D FP32, FP64, sloppy FP64; with the LDS halo (32): 4D FP32, sloppy FP64
+  const int nm = nd == 4 && (prec == F32 || prec == F64S) ? 5 : (nd == 4 && prec == F64) ? 4 : 3;
   float tmin[4][2][5];
   for (auto& a : tmin)
     for (auto& b : a)

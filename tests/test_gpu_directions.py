@@ -156,6 +156,7 @@ CASES = [
     (4, F64, 18, 16, (32, 32, 32, 64)),
     (4, F64, 18, 4, (32, 32, 32, 64)),
     (4, F64S, 18, 16, (32, 32, 32, 64)),
+    (4, F64S, 18, 32, (32, 32, 32, 64)),
     (4, F16, 12, 1, (16, 16, 16, 16)),
     (4, BF16, 18, 1, (16, 16, 16, 16)),
     (3, F32, 18, 1, (16, 16, 16, 8)),

@@ -311,8 +311,8 @@ def test_marching_lengths(oracle, nd, latt, prec, G):
     lib = _lib.core()
     b0, r0 = C.c_int(), C.c_int()
     _lib.check(lib.dslash_get_launch(nd, prec, G, C.byref(b0), C.byref(r0)))
-    marches = (1, 4, 8, 16, 32) if (nd == 4 and prec == F32) else \
-        (1, 4, 8, 16) if (nd == 4 and prec in (F64, F64S)) else (1, 4, 8)
+    marches = (1, 4, 8, 16, 32) if (nd == 4 and prec in (F32, F64S)) else \
+        (1, 4, 8, 16) if (nd == 4 and prec == F64) else (1, 4, 8)
     with _March(nd, prec, G) as m:
         for isign, cb in ((1, 0), (-1, 1)):
             want = oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, cb, nd=nd)
@@ -322,7 +322,10 @@ def test_marching_lengths(oracle, nd, latt, prec, G):
             for march in marches:
                 m.set(march)
                 _lib.check(lib.dslash_set_launch(nd, prec, G, 256 if march >= 16 else b0.value, r0.value))
-                assert _march_for(latt, nd, prec, G) == march
+                ran = _march_for(latt, nd, prec, G)
+                # 32 (box tiles with the LDS halo) falls back to 16 where the halo would
+                # need more than one entry per thread (64^3 x 2: 384 > 256)
+                assert ran == march or (march == 32 and ran == 16 and latt == (64, 64, 64, 2)), (march, ran)
                 d_res = op.pack_spinor(np.full_like(psi, 3.25))
                 op.apply(d_res, d_psi, isign, cb)
                 got = op.unpack_spinor(d_res, dtype=dt).reshape(-1, 24)
@@ -413,10 +416,11 @@ def test_xpay_epilogue_box_tiles(oracle, latt, prec, G):
                 t = slice(tcb * vh, (tcb + 1) * vh)
                 want = (x + a * oracle.dslash(u, psi, np.zeros_like(psi), latt, isign, 1 - tcb)).reshape(-1, 24)[t]
                 outs = {}
-                for march in ((32, 16, 1) if prec == F32 else (16, 1)):
+                for march in ((32, 16, 1) if prec in (F32, F64S) else (16, 1)):
                     m.set(march)
                     _lib.check(lib.dslash_set_launch(4, prec, G, 256, 1))
-                    assert _march_for(latt, 4, prec, G) == march
+                    ran = _march_for(latt, 4, prec, G)
+                    assert ran == march or (march == 32 and ran == 16 and latt == (64, 64, 64, 2)), (march, ran)
                     d_res = op.new_spinor(False)
                     _lib.check(lib.dslash_apply_xpay(ptr(d_res), ptr(d_psi), g, ptr(d_x), a, op._L, prec, G, isign,
                                                      tcb, stream))

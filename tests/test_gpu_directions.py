@@ -144,8 +144,8 @@ class Shape:
 CASES = [
     # (nd, prec, G, march, lattice): march 1 one site per thread, 4/8 marching,
     # 16 LDS-staged box tiles, 32 box tiles with the LDS halo (at 24 x 48^3 boxes of
-    # 12 x 3 x 6 sites, halo faces of unequal size); 32^3 x 64 is large enough for
-    # every shape to run
+    # 12 x 3 x 6 sites, halo faces of unequal size; boxes that span y or z have halo
+    # faces in the other direction only); 32^3 x 64 is large enough for every shape to run
     (4, F32, 18, 1, (32, 32, 32, 64)),
     (4, F32, 18, 8, (32, 32, 32, 64)),
     (4, F32, 18, 16, (32, 32, 32, 64)),
@@ -153,6 +153,8 @@ CASES = [
     (4, F32, 18, 32, (32, 32, 32, 64)),
     (4, F32, 12, 32, (32, 32, 32, 64)),
     (4, F32, 18, 32, (24, 48, 48, 48)),
+    (4, F32, 18, 32, (64, 4, 128, 64)),   # boxes span y: z faces only (halo 2 x 128)
+    (4, F32, 12, 32, (32, 64, 4, 256)),   # boxes span z: y faces only (halo 2 x 64)
     (4, F64, 18, 16, (32, 32, 32, 64)),
     (4, F64, 18, 4, (32, 32, 32, 64)),
     (4, F64S, 18, 16, (32, 32, 32, 64)),

@@ -285,6 +285,8 @@ MARCH_CASES = [(4, (32, 32, 32, 66), F64, 18), (4, (32, 32, 32, 66), F32, 12), (
                (4, (24, 48, 48, 48), F32, 18),  # box tiles of 12 x 3 x 6 sites: 40 idle lanes per block
                (4, (48, 48, 48, 24), F64, 18),  # box tiles of 24 x 3 x 3 sites: 40 idle lanes per block
                (4, (64, 64, 64, 2), F64S, 18), (4, (64, 64, 64, 2), F16, 12),
+               (4, (32, 128, 256, 2), F32, 18),  # halo tiles (16 x 4 x 4 boxes) on a 2-slice t extent
+               (4, (32, 2, 3280, 10), F64S, 18),  # 16 x 2 x 8 boxes span y: z faces only; 8 + 2 slices
                (3, (32, 32, 34, 65), F64, 12), (3, (32, 32, 34, 65), F32, 18), (3, (32, 32, 34, 65), F16, 18),
                (3, (64, 64, 2, 64), BF16, 18), (3, (64, 64, 2, 64), F64S, 12)]
 

@@ -93,8 +93,9 @@ struct BoxEpi {
 // res = D psi (or an epilogue form) for one FP32, FP64 or sloppy-FP64 4D field
 // pair with box tiles and non-temporal links (the single-RHS path's march 16);
 // needs box_tiles_single(latt, block) > 0 and block >= 128.
-// halo: FP32 tiles whose y / z hops that leave the box read an LDS halo of the
-// neighbouring sites, loaded one slice ahead (needs box_halo_single(latt, block) > 0).
+// halo: FP32 and sloppy-FP64 tiles whose y / z hops that leave the box read an LDS
+// halo of the neighbouring sites, loaded one slice ahead (needs
+// box_halo_single(latt, block) > 0; ignored for FP64, which has no halo form).
 int box_apply_single(void* res_cb, const void* psi_cb, const void* gauge_cb, const int latt[4], int prec, int G,
                      int isign, int tcb, int block, const BoxEpi& epi, hipStream_t s, bool halo = false);
 // Tiles (= blocks and |res|^2 partial sums) of that launch; 0 when Lx/2 > block,

@@ -348,7 +348,7 @@ template <int P> __device__ inline Spinor<typename Fmt<P>::compute> box_spinor(c
 // a backward face, (1 - isign gamma_mu) for a forward face.  Entries are
 // numbered [y backward | y forward | z backward | z forward], each face x-row
 // fastest; thread h of the block loads entry h.  Entries hold the 12 reals in
-// the compute type (3 vectors for FP32 and sloppy FP64, 6 for FP64).  LDS layout:
+// the compute type (3 vectors: the FP32 and sloppy-FP64 tiles both compute in FP32).  LDS layout:
 // [2 buffers][HV vectors][H] after the slab, buffer k & 1 holding slice t0 + k.
 struct BoxHalo {
   v4u* buf;  // buffer 0; buffer 1 at buf + HV H
@@ -706,7 +706,7 @@ __global__ void __launch_bounds__(256) k_box_halo(MrhsList list, BoxEpi e) {
   mrhs_box_grid<F32, G, 1, IS, true, EPI, true>(list, e);
 }
 
-// The same for one FP64 or sloppy-FP64 field (halo entries in the compute type).
+// The same for one sloppy-FP64 field: FP64 slab, halo entries in the FP32 compute type.
 template <int P, int G, int IS, int EPI>
 __global__ void __launch_bounds__(256) k_box_halo64(MrhsList list, BoxEpi e) {
   mrhs_box_grid<P, G, 1, IS, true, EPI, true>(list, e);
@@ -925,7 +925,7 @@ template <int P, int G, int IS>
 static void launch_box_single_s(const MrhsList& list, int block, const BoxEpi& e, hipStream_t s, bool halo) {
   const size_t lds = (size_t)2 * spinor_vec(P) * block * sizeof(v4u);
   const dim3 grid(list.total_tiles), blk(block);
-  {
+  if constexpr (P == F32 || P == F64S) {  // FP64 has no halo form: its tile already fills 256 VGPRs
     const MrhsLattice& L = list.lat[0];
     const int H = box_halo_entries(L.g.lxh, L.g.L[1], L.g.L[2], (int)L.ty, (int)L.tz);
     if (halo && H >= 1 && H <= block) {  // one halo entry per thread at most

@@ -91,7 +91,8 @@ struct BoxEpi {
   double* norm_partial;
 };
 // res = D psi (or an epilogue form) for one FP32, FP64 or sloppy-FP64 4D field
-// pair with box tiles and non-temporal links (the single-RHS path's march 16);
+// pair with box tiles and non-temporal links (the single-RHS path's march 16, and
+// march 32 with the halo below);
 // needs box_tiles_single(latt, block) > 0 and block >= 128.
 // halo: FP32 and sloppy-FP64 tiles whose y / z hops that leave the box read an LDS
 // halo of the neighbouring sites, loaded one slice ahead (needs

@@ -561,27 +561,28 @@ def cg_lattice_pmc_workload(reps: int = 10):
     log(f"[pmc-48] ceiling: {msf.value * 1e3:.1f} us/launch, {768 * vh} B/launch")
 
 
-def three_d_pmc_workload(reps: int = 12):
-    """For rocprofv3 --pmc runs: the 3D operators at 32^3 x 64 (FP32, FP64 and sloppy
-    FP64, 18-real links) in their default launch shapes (z-marching), 3 first-touch
-    launches and then reps launches each."""
+def operators_pmc_workload(reps: int = 12):
+    """For rocprofv3 --pmc runs: single-RHS operators at 32^3 x 64 with 18-real links
+    in their default launch shapes: 3D FP32, FP64 and sloppy FP64 (z-marching), then
+    4D FP64 (box tiles) and 4D sloppy FP64 (box tiles with the LDS halo); 3
+    first-touch launches and then reps launches each."""
     import ctypes as C
     import torch
     from sse_wilson_dslash_amd import F32, F64, F64S
     from sse_wilson_dslash_amd._lib import check
     from sse_wilson_dslash_amd.device import WilsonDslash
 
-    for prec in (F32, F64, F64S):
-        op = WilsonDslash(LATT, 3, prec, 18)
+    for nd, prec in ((3, F32), (3, F64), (3, F64S), (4, F64), (4, F64S)):
+        op = WilsonDslash(LATT, nd, prec, 18)
         b, r, m = C.c_int(), C.c_int(), C.c_int()
-        check(op.lib.dslash_get_launch(3, prec, 18, C.byref(b), C.byref(r)))
-        check(op.lib.dslash_get_launch_march(3, prec, 18, C.byref(m)))
+        check(op.lib.dslash_get_launch(nd, prec, 18, C.byref(b), C.byref(r)))
+        check(op.lib.dslash_launch_march_for(op._L, nd, prec, 18, C.byref(m)))
         op.gen_gauge(seed=1)
         psi = op.gen_spinor(seed=1)
         res = op.new_spinor(zero=True)
         op.time_apply(res, psi, 3)
         ms = op.time_apply(res, psi, reps)
-        log(f"[pmc-3d] prec={prec}: {ms * 1e3:.1f} us/launch, {op.bytes_per_call()} B/launch, "
+        log(f"[pmc-ops] nd={nd} prec={prec}: {ms * 1e3:.1f} us/launch, {op.bytes_per_call()} B/launch, "
             f"{b.value} threads, march {m.value}")
         del op, psi, res
         torch.cuda.empty_cache()
@@ -746,8 +747,9 @@ def main() -> int:
                     help="only the CG's operators and the ceiling kernel at 48^3x96 (for rocprofv3 --pmc runs)")
     ap.add_argument("--pmc-half", action="store_true",
                     help="only the 16-bit and FP32 one-site kernels in fixed shapes (for rocprofv3 --pmc runs)")
-    ap.add_argument("--pmc-3d", action="store_true",
-                    help="only the 3D FP32 / FP64 / sloppy-FP64 operators in their default shapes (for rocprofv3 --pmc runs)")
+    ap.add_argument("--pmc-ops", action="store_true",
+                    help="only the 3D FP32 / FP64 / sloppy-FP64 and 4D FP64 / sloppy-FP64 operators in their default "
+                         "shapes (for rocprofv3 --pmc runs)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="only a 2 GiB calibration copy and the headline kernel (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -775,8 +777,8 @@ def main() -> int:
     if args.pmc_cg_lattice:
         cg_lattice_pmc_workload()
         return 0
-    if args.pmc_3d:
-        three_d_pmc_workload()
+    if args.pmc_ops:
+        operators_pmc_workload()
         return 0
     if args.pmc_workload:
         copy_leg(2.0, 3)
